@@ -1,0 +1,79 @@
+/*
+ * scarf3d_srm.h - extensions of the MI355X SCARF3D SRM library beyond the
+ * reference API of scarf3d.h. None of these has a counterpart in the
+ * reference except where noted; all return 0 on success and a negative code
+ * on failure (details from scarf_last_error).
+ */
+#ifndef SCARF3D_SRM_H_
+#define SCARF3D_SRM_H_
+
+#include "scarf3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* field kernels */
+enum scarf_kernel {
+    SCARF_KERNEL_AUTO = 0,        /* mfma_bf16x3 on grids, direct_f32 on point lists (float lib); direct_f64 (double lib) */
+    SCARF_KERNEL_DIRECT_F32 = 1,  /* FP32 cos on FP64-reduced phases, FP64-promoted sums */
+    SCARF_KERNEL_DIRECT_F64 = 2,  /* FP64 throughout */
+    SCARF_KERNEL_MFMA_BF16X3 = 3, /* grid GEMM, 3-term bf16 split, 6 MFMA products */
+    SCARF_KERNEL_MFMA_F16X2 = 4,  /* grid GEMM, scaled 2-term fp16 split, 3 MFMA products */
+    SCARF_KERNEL_MFMA_BF16X2 = 5  /* grid GEMM, 2-term bf16 split, 3 MFMA products (about 1e-5 sigma) */
+};
+
+/* Number of harmonics of the spectral sum. The reference fixes it at compile
+ * time to 10000 (src/scarflib_srm.f90:90); that stays the default. */
+int scarf_set_nharm(int nharm);
+int scarf_get_nharm(void);
+
+/* Field kernel for subsequent executes (environment SCARF_KERNEL=name also
+ * works: auto, direct_f32, direct_f64, mfma_bf16x3, mfma_f16x2, mfma_bf16x2). */
+int scarf_set_kernel(int kernel);
+int scarf_get_kernel(void);
+
+/* Last error message (empty string when the last call succeeded). */
+const char *scarf_last_error(void);
+
+/* scarf_execute writing a device-resident field (d_field: device pointer
+ * with room for the local block or point list); no host copy of the field. */
+int scarf_execute_device(const int seed, real *d_field, real stats[]);
+
+/* Two halves of scarf_execute_device, so that the device work can be timed
+ * with the harmonic table already resident: scarf_srm_draw samples the
+ * harmonics of `seed` on the host and uploads them (host seconds in
+ * *host_seconds); scarf_srm_run runs tables + field + statistics + rescale
+ * + taper on the resident table. Returns the number of harmonics kept
+ * (scarf_srm_draw) or 0. */
+int scarf_srm_draw(const int seed, double *host_seconds);
+int scarf_srm_run(real *d_field, real stats[]);
+
+/* Device milliseconds of the last run: {tables, field, statistics,
+ * epilogue, total}; returns the number written (<= n). */
+int scarf_last_timings(double *ms, int n);
+
+/* HIP stream the library launches on (hipStream_t as void*). */
+void *scarf_stream(void);
+
+/* Setup values of the current configuration (for checks):
+ * out[0..8] = {ndim, kmin, kmax, kc, info1, info2, scaling, intg(kmin,kmax),
+ * intg()}, out[9..17] = rotation matrix (column-major), out[18..20] = bar. */
+int scarf_setup_values(double *out, int n);
+
+/* Parameters held after initialize (defaults applied), for checks:
+ * {nd, structured, fs[3], fe[3], ds, dh, acf, cl[3], sigma, hurst, nc[3],
+ * fc[3], alpha, beta, gamma, mute, taper, rescale, pad, method, npoi, npts};
+ * returns the number written (<= n). */
+int scarf_get_config(double *out, int n);
+
+/* Harmonics of `seed` as drawn on the host: harm (nharm x 5, row-major
+ * {v1, v2, v3, a, b}) and used[nharm] (0 = dropped by u > kc). Returns the
+ * number of uniforms consumed. */
+long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

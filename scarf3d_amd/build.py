@@ -1,0 +1,98 @@
+"""Build recipe of the SRM library for gfx950 (hipcc, in-tree).
+
+Produces scarf3d_amd/lib/libscarf3df.so (real = float) and
+scarf3d_amd/lib/libscarf3d.so (real = double, -DDOUBLE_PREC), the two library
+flavours of the reference (src/Makefile:9-13). The kernels are compiled once
+(they are templated on the field type); only the C ABI layer differs.
+
+Usage: python -m scarf3d_amd.build [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+CSRC = HERE / "csrc"
+BUILD = HERE / "_build"
+LIB = HERE / "lib"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", str(CSRC), "-I", str(ROOT / "include")]
+# Packed FP32 instructions are kept off for every HIP translation unit.
+DEVICE = [f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# The sampler reproduces the reference's roundings: no FMA contraction, and
+# sin/cos stay separate libm calls (a fused sincos may differ in the last bit).
+HOST_EXACT = ["-O2", "-ffp-contract=off", "-fno-fast-math", "-fno-builtin-sin", "-fno-builtin-cos",
+              "-fno-builtin-sinf", "-fno-builtin-cosf", "-fno-builtin-sincos", "-fno-builtin-sincosf"]
+
+UNITS = [
+    # (source, extra flags, object name)
+    ("srm_kernels.hip", DEVICE, "srm_kernels.o"),
+    ("srm_mfma.hip", DEVICE, "srm_mfma.o"),
+    ("srm_host.cpp", DEVICE + HOST_EXACT, "srm_host.o"),
+    ("quadpack.cpp", DEVICE + HOST_EXACT, "quadpack.o"),
+    ("scarf_capi.cpp", DEVICE + HOST_EXACT, "scarf_capi_f.o"),
+    ("scarf_capi.cpp", DEVICE + HOST_EXACT + ["-DDOUBLE_PREC"], "scarf_capi_d.o"),
+]
+
+
+def _compile(src: str, flags: list[str], obj: str, verbose: bool) -> None:
+    cmd = [HIPCC, "-x", "hip", *COMMON, *flags, "-c", str(CSRC / src), "-o", str(BUILD / obj)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    msgs = [ln for ln in (res.stdout + res.stderr).splitlines() if noise not in ln]
+    if res.returncode != 0:
+        raise RuntimeError(f"compiling {src} failed:\n" + "\n".join(msgs))
+    if msgs and verbose:
+        print("\n".join(msgs))
+
+
+def _link(objs: list[str], out: Path, verbose: bool) -> None:
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(out), *[str(BUILD / o) for o in objs]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"linking {out.name} failed:\n{res.stdout}{res.stderr}")
+
+
+def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
+    """Compile every HIP translation unit for gfx950 and link both libraries."""
+    BUILD.mkdir(exist_ok=True)
+    LIB.mkdir(exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_compile, s, f, o, verbose) for s, f, o in UNITS]
+        for fut in futs:
+            fut.result()
+    shared = ["srm_kernels.o", "srm_mfma.o", "srm_host.o", "quadpack.o"]
+    outs = []
+    for obj, name in (("scarf_capi_f.o", "libscarf3df.so"), ("scarf_capi_d.o", "libscarf3d.so")):
+        out = LIB / name
+        tmp = LIB / (name + ".tmp")
+        _link([*shared, obj], tmp, verbose)
+        os.replace(tmp, out)
+        outs.append(out)
+    return outs
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("-v", "--verbose", action="store_true")
+    args = ap.parse_args(argv)
+    for p in build(args.jobs, args.verbose):
+        print(p)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

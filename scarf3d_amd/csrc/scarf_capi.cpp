@@ -1,0 +1,572 @@
+// C ABI of the library and the object layer behind it.
+//
+// One process-global object, as in the reference (`obj`, src/scarflib.f90:87,
+// and the binding's `dims`/`structured`, src/f_binding.f90:44-47): initialize
+// stores the parameters with their defaults, execute runs one realization,
+// finalize releases the state. The library is not re-entrant, like the
+// reference. Differences, all deliberate:
+//   - points of interest are copied at initialize (the reference keeps a
+//     pointer, and its unstructured path assigns into a zero-size pointer);
+//   - errors are reported (stderr + scarf_last_error) instead of crashing:
+//     initialize twice, execute before initialize, FIM (method 0), acf = 2,
+//     von Karman with hurst <= 0;
+//   - nharm is a runtime setting (default 10000).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/scarf3d.h"
+#include "../../include/scarf3d_srm.h"
+#include "srm_device.hpp"
+#include "srm_host.hpp"
+
+namespace {
+
+using scarf::SrmConfig;
+
+struct State {
+    bool initialized = false;
+    SrmConfig cfg;
+    const real *x = nullptr, *y = nullptr, *z = nullptr;  // borrowed point coordinates
+    int kernel = -1;                                       // -1: not chosen yet (environment / auto)
+    int nharm = 10000;
+    std::string error;
+    std::unique_ptr<scarf::Device> device;
+    // resident realization (scarf_srm_draw -> scarf_srm_run)
+    bool table_ready = false;
+    scarf::SrmSetup setup;
+    double host_seconds = 0;
+};
+
+State &state()
+{
+    static State s;
+    return s;
+}
+
+int fail(const std::string &msg)
+{
+    State &s = state();
+    s.error = msg;
+    std::fprintf(stderr, "scarf3d: error: %s\n", msg.c_str());
+    return -1;
+}
+
+constexpr bool kDouble = sizeof(real) == sizeof(double);
+
+int kernel_from_env()
+{
+    const char *e = std::getenv("SCARF_KERNEL");
+    if (!e || !*e) return scarf::kKernelAuto;
+    for (int k = 0; k <= scarf::kKernelMfmaBf16x2; ++k)
+        if (std::strcmp(e, scarf::kernel_name(k)) == 0) return k;
+    std::fprintf(stderr, "scarf3d: warning: unknown SCARF_KERNEL=%s, using auto\n", e);
+    return scarf::kKernelAuto;
+}
+
+int current_kernel()
+{
+    State &s = state();
+    if (s.kernel < 0) s.kernel = kernel_from_env();
+    return s.kernel;
+}
+
+// kernel actually used for the current configuration
+int resolve_kernel(const SrmConfig &cfg)
+{
+    const int k = current_kernel();
+    if (k != scarf::kKernelAuto) return k;
+    if (kDouble) return scarf::kKernelDirectF64;
+    if (!cfg.structured) return scarf::kKernelDirectF32;
+    const long nx = cfg.fe[0] - cfg.fs[0] + 1, ny = cfg.fe[1] - cfg.fs[1] + 1, nz = cfg.fe[2] - cfg.fs[2] + 1;
+    // the GEMM needs enough 256 x 128 tiles to fill 256 compute units
+    const long tiles = ((nx + 127) / 128) * ((ny * nz + 255) / 256);
+    return tiles >= 512 ? scarf::kKernelMfmaBf16x3 : scarf::kKernelDirectF32;
+}
+
+scarf::Geometry geometry(const SrmConfig &cfg)
+{
+    scarf::Geometry g;
+    g.structured = cfg.structured;
+    if (cfg.structured) {
+        g.nx = cfg.fe[0] - cfg.fs[0] + 1;
+        g.ny = cfg.fe[1] - cfg.fs[1] + 1;
+        g.nz = cfg.fe[2] - cfg.fs[2] + 1;
+        g.ox = cfg.fs[0] - 1;
+        g.oy = cfg.fs[1] - 1;
+        g.oz = cfg.fs[2] - 1;
+        g.ds = static_cast<real>(cfg.ds);
+    } else {
+        g.npts = cfg.npts;
+    }
+    return g;
+}
+
+scarf::Epilogue epilogue(const SrmConfig &cfg, int ndim)
+{
+    scarf::Epilogue ep;
+    ep.rescale = cfg.rescale;
+    ep.sigma = cfg.sigma;
+    ep.mute = cfg.mute;
+    ep.taper = cfg.taper;
+    ep.ndim = ndim;
+    // pt(1:ndim) = poi(1:ndim, i), pt(3) = 0 in 2D (src/scarflib_srm.f90:920-925)
+    for (int q = 0; q < cfg.npoi; ++q) {
+        double p3[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < ndim && d < cfg.nd; ++d) p3[d] = cfg.poi[size_t(q) * cfg.nd + d];
+        ep.poi3.insert(ep.poi3.end(), p3, p3 + 3);
+    }
+    return ep;
+}
+
+bool ensure_device()
+{
+    State &s = state();
+    if (!s.device) s.device = std::make_unique<scarf::Device>();
+    std::string err;
+    if (!s.device->ensure(err)) {
+        fail(err);
+        return false;
+    }
+    return true;
+}
+
+void clear_stats(real stats[])
+{
+    if (stats)
+        for (int i = 0; i < 8; ++i) stats[i] = 0;
+}
+
+// host half of a realization: setup, draw, upload
+int prepare(int seed)
+{
+    State &s = state();
+    s.table_ready = false;
+    if (!s.initialized) return fail("execute called before initialize");
+    if (s.cfg.method != 1) return fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
+    s.cfg.nharm = s.nharm;
+    std::string err;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, err) : scarf::srm_setup<float>(s.cfg, s.setup, err);
+    if (rc != 0) return fail(err);
+    if (!ensure_device()) return -1;
+    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, s.setup, seed)
+                                          : scarf::srm_draw<float>(s.cfg, s.setup, seed);
+    const scarf::HarmonicTable t = scarf::build_table(s.setup, d);
+    s.host_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!s.device->upload_table(t, err)) return fail(err);
+    if (!s.cfg.structured) {
+        if (!s.device->upload_points<real>(s.x, s.y, s.z, s.cfg.npts, err)) return fail(err);
+    }
+    s.table_ready = true;
+    s.error.clear();
+    return d.n_used;
+}
+
+// device half: tables, field, statistics, epilogue into d_field
+int run(real *d_field, real stats[])
+{
+    State &s = state();
+    if (!s.table_ready) return fail("no harmonic table drawn (call scarf_srm_draw first)");
+    const scarf::Geometry g = geometry(s.cfg);
+    const scarf::Epilogue ep = epilogue(s.cfg, s.setup.ndim);
+    const int kernel = resolve_kernel(s.cfg);
+    double st[6] = {};
+    std::string err;
+    if (!s.device->run<real>(g, kernel, s.setup.scaling, ep, d_field, st, err)) return fail(err);
+    if (stats) {
+        clear_stats(stats);
+        stats[0] = static_cast<real>(s.setup.info1);
+        stats[1] = static_cast<real>(s.setup.info2);
+        // rescale reports sigma and zero mean (src/scarflib_srm.f90:909-910)
+        stats[2] = s.cfg.rescale == 1 ? static_cast<real>(s.cfg.sigma) : static_cast<real>(st[3]);
+        stats[3] = s.cfg.rescale == 1 ? real(0) : static_cast<real>(st[4]);
+        stats[4] = static_cast<real>(s.host_seconds);
+        stats[5] = static_cast<real>(s.device->timings().total * 1e-3);
+    }
+    s.error.clear();
+    return 0;
+}
+
+long field_points(const SrmConfig &cfg)
+{
+    if (!cfg.structured) return cfg.npts;
+    return long(cfg.fe[0] - cfg.fs[0] + 1) * (cfg.fe[1] - cfg.fs[1] + 1) * (cfg.fe[2] - cfg.fs[2] + 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+void scarf_opt_init(struct scarf_opt *var)
+{
+    if (!var) return;
+    var->method = 0;
+    var->hurst = 0;
+    var->dh = 0;
+    var->ds = 0;
+    var->poi = nullptr;
+    var->npoi = 0;
+    var->mute = 0;
+    var->taper = 0;
+    var->rescale = 0;
+    var->pad = 0;
+    var->nc = nullptr;
+    var->fc = nullptr;
+    var->alpha = 0;
+    var->beta = 0;
+    var->gamma = 0;
+}
+
+// initialize_structured (src/scarflib.f90:97-189) behind struct_initialize
+// (src/f_binding.f90:57-113) and the option rules of src/c_binding.c:89-103
+void scarf_cart_initialize(const int nd, const int fs[], const int fe[], const real ds, const int acf,
+                           const real cl[], const real sigma, struct scarf_opt *var)
+{
+    State &s = state();
+    if (s.initialized) {
+        fail("initialize called twice without finalize");
+        return;
+    }
+    if ((nd != 2 && nd != 3) || !fs || !fe || !cl) {
+        fail("scarf_cart_initialize: nd must be 2 or 3 and fs, fe, cl non-NULL");
+        return;
+    }
+    SrmConfig c;
+    c.nd = nd;
+    c.structured = true;
+    for (int i = 0; i < 3; ++i) {
+        c.fs[i] = 1;
+        c.fe[i] = 1;
+        c.cl[i] = 0;
+    }
+    for (int i = 0; i < nd; ++i) {
+        c.fs[i] = fs[i];
+        c.fe[i] = fe[i];
+        c.cl[i] = cl[i];
+        if (fe[i] < fs[i]) {
+            fail("scarf_cart_initialize: fe < fs");
+            return;
+        }
+    }
+    c.ds = ds;
+    c.dh = ds;
+    c.acf = acf;
+    c.sigma = sigma;
+    c.method = 0;
+    c.hurst = 0;
+    c.mute = -999;
+    c.taper = -999;
+    c.rescale = 0;
+    c.pad = 0;
+    for (int i = 0; i < 3; ++i) {
+        c.nc[i] = static_cast<real>(static_cast<real>(c.fs[i] - 1) * ds);
+        c.fc[i] = static_cast<real>(static_cast<real>(c.fe[i] - 1) * ds);
+    }
+    if (var) {
+        if (var->method == 1) c.method = 1;
+        if (var->hurst > 0) c.hurst = var->hurst;
+        if (var->dh > 0) c.dh = var->dh;
+        if (var->mute > 0) c.mute = var->mute;
+        if (var->taper > 0) c.taper = var->taper;
+        if (var->rescale == 1) c.rescale = 1;
+        if (var->pad == 1) c.pad = 1;
+        if (var->nc && var->fc)
+            for (int i = 0; i < nd; ++i) {
+                c.nc[i] = var->nc[i];
+                c.fc[i] = var->fc[i];
+            }
+        if (var->alpha != 0) c.alpha = var->alpha;
+        if (var->beta != 0) c.beta = var->beta;
+        if (var->gamma != 0) c.gamma = var->gamma;
+        if (var->poi && var->npoi > 0) {
+            c.npoi = var->npoi;
+            c.poi.assign(var->poi, var->poi + size_t(nd) * var->npoi);
+        }
+    }
+    s.cfg = c;
+    s.x = s.y = s.z = nullptr;
+    s.initialized = true;
+    s.table_ready = false;
+    s.error.clear();
+}
+
+// initialize_unstructured (src/scarflib.f90:195-296) behind unstruct_initialize
+// (src/f_binding.f90:119-175) and src/c_binding.c:116-130
+void scarf_nocart_initialize(const int nd, const int npts, const real *x, const real *y, const real *z,
+                             const real dh, const int acf, const real cl[], const real sigma, struct scarf_opt *var)
+{
+    State &s = state();
+    if (s.initialized) {
+        fail("initialize called twice without finalize");
+        return;
+    }
+    if ((nd != 2 && nd != 3) || npts < 1 || !x || !y || !cl || (nd == 3 && !z)) {
+        fail("scarf_nocart_initialize: nd must be 2 or 3, npts > 0, x/y (and z in 3D) and cl non-NULL");
+        return;
+    }
+    SrmConfig c;
+    c.nd = nd;
+    c.structured = false;
+    c.npts = npts;
+    for (int i = 0; i < 3; ++i) c.cl[i] = 0;
+    for (int i = 0; i < nd; ++i) c.cl[i] = cl[i];
+    c.dh = dh;
+    c.ds = dh;
+    c.acf = acf;
+    c.sigma = sigma;
+    c.method = 0;
+    c.hurst = 0;
+    c.mute = -999;
+    c.taper = -999;
+    c.rescale = 0;
+    c.pad = 0;
+    const real *axes[3] = {x, y, nd == 3 ? z : nullptr};
+    for (int a = 0; a < 3; ++a) {
+        if (!axes[a]) {
+            c.nc[a] = 0;
+            c.fc[a] = 0;
+            continue;
+        }
+        real lo = axes[a][0], hi = axes[a][0];
+        for (int i = 1; i < npts; ++i) {
+            lo = std::min(lo, axes[a][i]);
+            hi = std::max(hi, axes[a][i]);
+        }
+        c.nc[a] = lo;
+        c.fc[a] = hi;
+    }
+    if (var) {
+        if (var->method == 1) c.method = 1;
+        if (var->hurst > 0) c.hurst = var->hurst;
+        if (var->ds > 0) c.ds = var->ds;
+        if (var->mute > 0) c.mute = var->mute;
+        if (var->taper > 0) c.taper = var->taper;
+        if (var->rescale == 1) c.rescale = 1;
+        if (var->pad == 1) c.pad = 1;
+        if (var->nc && var->fc)
+            for (int i = 0; i < nd; ++i) {
+                c.nc[i] = var->nc[i];
+                c.fc[i] = var->fc[i];
+            }
+        if (var->alpha != 0) c.alpha = var->alpha;
+        if (var->beta != 0) c.beta = var->beta;
+        if (var->gamma != 0) c.gamma = var->gamma;
+        if (var->poi && var->npoi > 0) {
+            c.npoi = var->npoi;
+            c.poi.assign(var->poi, var->poi + size_t(nd) * var->npoi);
+        }
+    }
+    s.cfg = c;
+    s.x = x;
+    s.y = y;
+    s.z = nd == 3 ? z : nullptr;
+    s.initialized = true;
+    s.table_ready = false;
+    s.error.clear();
+}
+
+void scarf_execute(const int seed, real *field, real stats[])
+{
+    State &s = state();
+    clear_stats(stats);
+    if (!field) {
+        fail("scarf_execute: field is NULL");
+        return;
+    }
+    if (prepare(seed) < 0) return;
+    const long n = field_points(s.cfg);
+    std::string err;
+    void *d = s.device->field_buffer(size_t(n) * sizeof(real), err);
+    if (!d) {
+        fail(err);
+        return;
+    }
+    if (run(static_cast<real *>(d), stats) != 0) return;
+    const hipError_t e = hipMemcpy(field, d, size_t(n) * sizeof(real), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) fail(std::string("copying the field to the host: ") + hipGetErrorString(e));
+}
+
+void scarf_finalize(void)
+{
+    State &s = state();
+    s.initialized = false;
+    s.table_ready = false;
+    s.cfg = SrmConfig{};
+    s.x = s.y = s.z = nullptr;
+    s.error.clear();
+}
+
+// Single-writer versions of write_one / write_slice (src/scarflib_common.f90:245-418,
+// :527-646): raw native-endian `real`, column-major (x fastest).
+void scarf_io_one(const int nd, const int *npts, const real *field, const char fname[], const int *nwriters)
+{
+    (void)nwriters;
+    if (!npts || !field || !fname || (nd != 2 && nd != 3)) {
+        fail("scarf_io_one: bad arguments");
+        return;
+    }
+    size_t n = 1;
+    for (int i = 0; i < nd; ++i) n *= size_t(npts[i]);
+    FILE *f = std::fopen(fname, "wb");
+    if (!f || std::fwrite(field, sizeof(real), n, f) != n) fail(std::string("scarf_io_one: cannot write ") + fname);
+    if (f) std::fclose(f);
+}
+
+void scarf_io_slice(const int *npts, const char axis[], const int plane, const real *field, const char fname[])
+{
+    if (!npts || !axis || !field || !fname) {
+        fail("scarf_io_slice: bad arguments");
+        return;
+    }
+    const long nx = npts[0], ny = npts[1], nz = npts[2];
+    std::vector<real> out;
+    const long p = plane - 1;  // plane is 1-based
+    if (axis[0] == 'x') {
+        if (p < 0 || p >= nx) return (void)fail("scarf_io_slice: plane out of range");
+        for (long k = 0; k < nz; ++k)
+            for (long j = 0; j < ny; ++j) out.push_back(field[p + nx * (j + ny * k)]);
+    } else if (axis[0] == 'y') {
+        if (p < 0 || p >= ny) return (void)fail("scarf_io_slice: plane out of range");
+        for (long k = 0; k < nz; ++k)
+            for (long i = 0; i < nx; ++i) out.push_back(field[i + nx * (p + ny * k)]);
+    } else if (axis[0] == 'z') {
+        if (p < 0 || p >= nz) return (void)fail("scarf_io_slice: plane out of range");
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) out.push_back(field[i + nx * (j + ny * p)]);
+    } else {
+        fail("scarf_io_slice: axis must be x, y or z");
+        return;
+    }
+    FILE *f = std::fopen(fname, "wb");
+    if (!f || std::fwrite(out.data(), sizeof(real), out.size(), f) != out.size())
+        fail(std::string("scarf_io_slice: cannot write ") + fname);
+    if (f) std::fclose(f);
+}
+
+// ---- extensions (scarf3d_srm.h) ----
+int scarf_set_nharm(int nharm)
+{
+    if (nharm < 1) return fail("nharm must be positive");
+    state().nharm = nharm;
+    state().table_ready = false;
+    return 0;
+}
+
+int scarf_get_nharm(void) { return state().nharm; }
+
+int scarf_set_kernel(int kernel)
+{
+    if (kernel < scarf::kKernelAuto || kernel > scarf::kKernelMfmaBf16x2) return fail("unknown kernel");
+    state().kernel = kernel;
+    return 0;
+}
+
+int scarf_get_kernel(void)
+{
+    State &s = state();
+    return s.initialized ? resolve_kernel(s.cfg) : current_kernel();
+}
+
+const char *scarf_last_error(void) { return state().error.c_str(); }
+
+int scarf_execute_device(const int seed, real *d_field, real stats[])
+{
+    clear_stats(stats);
+    if (!d_field) return fail("scarf_execute_device: d_field is NULL");
+    if (prepare(seed) < 0) return -1;
+    return run(d_field, stats);
+}
+
+int scarf_srm_draw(const int seed, double *host_seconds)
+{
+    const int n = prepare(seed);
+    if (host_seconds) *host_seconds = state().host_seconds;
+    return n;
+}
+
+int scarf_srm_run(real *d_field, real stats[])
+{
+    if (!d_field) return fail("scarf_srm_run: d_field is NULL");
+    return run(d_field, stats);
+}
+
+int scarf_last_timings(double *ms, int n)
+{
+    State &s = state();
+    if (!s.device || !ms) return 0;
+    const scarf::Timings &t = s.device->timings();
+    const double v[5] = {t.tables, t.field, t.stats, t.epilogue, t.total};
+    const int m = std::min(n, 5);
+    for (int i = 0; i < m; ++i) ms[i] = v[i];
+    return m;
+}
+
+void *scarf_stream(void)
+{
+    if (!ensure_device()) return nullptr;
+    return reinterpret_cast<void *>(state().device->stream());
+}
+
+int scarf_setup_values(double *out, int n)
+{
+    State &s = state();
+    if (!s.initialized) return fail("not initialized");
+    s.cfg.nharm = s.nharm;
+    scarf::SrmSetup su;
+    std::string err;
+    const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, su, err) : scarf::srm_setup<float>(s.cfg, su, err);
+    if (rc != 0) return fail(err);
+    double v[21] = {double(su.ndim), su.kmin, su.kmax, su.kc, su.info1, su.info2, su.scaling, su.band, su.all};
+    for (int i = 0; i < 9; ++i) v[9 + i] = su.matrix[i];
+    for (int i = 0; i < 3; ++i) v[18 + i] = su.bar[i];
+    for (int i = 0; i < std::min(n, 21); ++i) out[i] = v[i];
+    return 0;
+}
+
+int scarf_get_config(double *out, int n)
+{
+    State &s = state();
+    if (!s.initialized) return fail("not initialized");
+    const SrmConfig &c = s.cfg;
+    const double v[32] = {double(c.nd), double(c.structured), double(c.fs[0]), double(c.fs[1]), double(c.fs[2]),
+                          double(c.fe[0]), double(c.fe[1]), double(c.fe[2]), c.ds, c.dh, double(c.acf), c.cl[0],
+                          c.cl[1], c.cl[2], c.sigma, c.hurst, c.nc[0], c.nc[1], c.nc[2], c.fc[0], c.fc[1], c.fc[2],
+                          c.alpha, c.beta, c.gamma, c.mute, c.taper, double(c.rescale), double(c.pad),
+                          double(c.method), double(c.npoi), double(c.npts)};
+    const int m = std::min(n, 32);
+    for (int i = 0; i < m; ++i) out[i] = v[i];
+    return m;
+}
+
+long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
+{
+    State &s = state();
+    if (!s.initialized) return fail("not initialized");
+    s.cfg.nharm = s.nharm;
+    scarf::SrmSetup su;
+    std::string err;
+    const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, su, err) : scarf::srm_setup<float>(s.cfg, su, err);
+    if (rc != 0) return fail(err);
+    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, su, seed)
+                                          : scarf::srm_draw<float>(s.cfg, su, seed);
+    for (int l = 0; l < s.cfg.nharm; ++l) {
+        harm[5 * l] = d.v1[l];
+        harm[5 * l + 1] = d.v2[l];
+        harm[5 * l + 2] = d.v3[l];
+        harm[5 * l + 3] = d.a[l];
+        harm[5 * l + 4] = d.b[l];
+        used[l] = d.used[l];
+    }
+    return d.uniforms;
+}
+
+}  // extern "C"

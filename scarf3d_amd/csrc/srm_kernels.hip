@@ -1,0 +1,690 @@
+// gfx950 kernels of the SRM direct sum and its epilogue, and the launch
+// sequence that ties them together (Device::run).
+//
+// Reference semantics (src/scarflib_srm.f90:794-851, :397-436):
+//   field(p) = scaling * sum_n [a_n sin(arg_n(p)) + b_n cos(arg_n(p))],
+//   arg_n(p) = v_n . (M p + bar).
+// The host rewrites each harmonic as amp_n cos(w_n . p + phase_n) (srm_host).
+// On a Cartesian grid the phase splits per axis,
+//   w.p + phase = wx*x_i + wy*y_j + (wz*z_k + phase),
+// so three small tables of phases, reduced modulo one turn in FP64, replace
+// the per-point FP32 products that lose ~1e-4 rad at |arg| ~ 3e3 rad. The
+// FP32 kernel then spends one add, one v_cos_f32 (argument in turns) and one
+// FMA per point-harmonic; partial sums are promoted to FP64 every 128
+// harmonics so that FP32 rounding does not grow with the harmonic count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "srm_device.hpp"
+#include "srm_mfma.hpp"
+#include "srm_stats.hpp"
+
+namespace scarf {
+
+namespace {
+
+constexpr double kInvTwoPi = 0.15915494309189533577;  // 1 / (2 pi)
+constexpr int kHarmPad = 64;                          // harmonic count padding
+
+__host__ __device__ inline long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+// Final merge of the per-block partials in a fixed order, then the `real`
+// roundings of the reference: info(3) = sqrt(real(var)), info(4) = real(mean),
+// rescale factor sigma / info(3) (src/scarflib_srm.f90:880-899).
+template <class Real>
+__global__ __launch_bounds__(256) void stats_finalize(const double *__restrict__ part, int nparts,
+                                                      double sigma, double *__restrict__ stat)
+{
+    __shared__ double sm[3][256];
+    Moments m{0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += 256) merge_moments(m, Moments{part[3 * b], part[3 * b + 1], part[3 * b + 2]});
+    sm[0][threadIdx.x] = m.n;
+    sm[1][threadIdx.x] = m.mean;
+    sm[2][threadIdx.x] = m.m2;
+    __syncthreads();
+    for (int stride = 128; stride >= 1; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            Moments a{sm[0][threadIdx.x], sm[1][threadIdx.x], sm[2][threadIdx.x]};
+            merge_moments(a, Moments{sm[0][threadIdx.x + stride], sm[1][threadIdx.x + stride], sm[2][threadIdx.x + stride]});
+            sm[0][threadIdx.x] = a.n;
+            sm[1][threadIdx.x] = a.mean;
+            sm[2][threadIdx.x] = a.m2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = sm[0][0], mean = sm[1][0], m2 = sm[2][0];
+        const double var = n > 1.0 ? m2 / (n - 1.0) : 0.0;
+        const Real info3 = sqrt(static_cast<Real>(var));
+        const Real info4 = static_cast<Real>(mean);
+        const Real sc = static_cast<Real>(sigma) / info3;
+        stat[0] = n;
+        stat[1] = mean;
+        stat[2] = var;
+        stat[3] = info3;
+        stat[4] = info4;
+        stat[5] = sc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-axis phase tables, in turns, reduced to [-1/2, 1/2] in FP64
+//   tx[n][i] = frac(wx_n * x_i / 2pi), ty[n][j] likewise,
+//   tz[n][k] = frac((wz_n * z_k + phase_n) / 2pi)
+template <class T>
+__global__ __launch_bounds__(256) void axis_tables(const double *__restrict__ harm, int np_, int nx, int ny,
+                                                   int nz, int ox, int oy, int oz, double ds,
+                                                   T *__restrict__ tx, T *__restrict__ ty, T *__restrict__ tz,
+                                                   float *__restrict__ amp32)
+{
+    const long total = long(np_) * (nx + ny + nz);
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < total; e += long(gridDim.x) * blockDim.x) {
+        const int n = int(e / (nx + ny + nz));
+        const int r = int(e % (nx + ny + nz));
+        double t;
+        if (r < nx) {
+            t = (harm[n] * (double(r + ox) * ds)) * kInvTwoPi;
+            tx[long(n) * nx + r] = static_cast<T>(t - rint(t));
+        } else if (r < nx + ny) {
+            const int j = r - nx;
+            t = (harm[np_ + n] * (double(j + oy) * ds)) * kInvTwoPi;
+            ty[long(n) * ny + j] = static_cast<T>(t - rint(t));
+        } else {
+            const int k = r - nx - ny;
+            t = (harm[2 * np_ + n] * (double(k + oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
+            tz[long(n) * nz + k] = static_cast<T>(t - rint(t));
+        }
+        if (amp32 && r == 0) amp32[n] = static_cast<float>(harm[4 * np_ + n]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 direct sum on a Cartesian grid.
+// Block tile: 64 x-points by 64 columns (column c = j + ny*k), 256 threads,
+// each thread 4 consecutive x-points by 4 columns. Harmonics stream through
+// LDS 32 at a time: the tile's x-phases, the tile's (y+z)-phases (summed and
+// re-reduced once per column-harmonic) and the amplitudes.
+constexpr int kDirH = 32;
+constexpr int kDirB = 64;
+
+template <class Out>
+__global__ __launch_bounds__(256) void direct_grid_f32(const float *__restrict__ tx, const float *__restrict__ ty,
+                                                       const float *__restrict__ tz,
+                                                       const float *__restrict__ amp, int np_, int nx, int ny,
+                                                       int nz, int tiles_i, double scaling,
+                                                       Out *__restrict__ field, double *__restrict__ part)
+{
+    __shared__ float4 sx[kDirH][kDirB / 4];
+    __shared__ float4 syz[kDirH][kDirB / 4];
+    __shared__ float sa[kDirH];
+    const int tid = threadIdx.x;
+    const int ti = tid & 15, tc = tid >> 4;
+    const int i0 = (blockIdx.x % tiles_i) * kDirB;
+    const long c0 = long(blockIdx.x / tiles_i) * kDirB;
+    const long ncols = long(ny) * nz;
+
+    // loader role: one x-point / one column, harmonic rows lh, lh+8, ...
+    const int lcol = tid & 63, lh = tid >> 6;
+    const long lc = c0 + lcol;
+    const bool lc_ok = lc < ncols;
+    const int lj = lc_ok ? int(lc % ny) : 0, lk = lc_ok ? int(lc / ny) : 0;
+    const int li = i0 + lcol;
+    const bool li_ok = li < nx;
+    float *sxf = reinterpret_cast<float *>(sx);
+    float *syzf = reinterpret_cast<float *>(syz);
+
+    float acc[4][4];
+    double acc64[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = 0.f;
+            acc64[a][b] = 0.0;
+        }
+
+    int tile = 0;
+    for (int n0 = 0; n0 < np_; n0 += kDirH, ++tile) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kDirH / 4; ++q) {
+            const int h = lh + 4 * q;
+            const long n = n0 + h;
+            sxf[h * kDirB + lcol] = li_ok ? tx[n * nx + li] : 0.f;
+            float t = 0.f;
+            if (lc_ok) {
+                t = ty[n * ny + lj] + tz[n * nz + lk];
+                t -= rintf(t);
+            }
+            syzf[h * kDirB + lcol] = t;
+        }
+        if (tid < kDirH) sa[tid] = amp[n0 + tid];
+        __syncthreads();
+#pragma unroll 4
+        for (int h = 0; h < kDirH; ++h) {
+            const float4 xv = sx[h][ti];
+            const float4 yv = syz[h][tc];
+            const float a = sa[h];
+            const float xs[4] = {xv.x, xv.y, xv.z, xv.w};
+            const float ys[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[p][c] = fmaf(a, __builtin_amdgcn_cosf(xs[p] + ys[c]), acc[p][c]);
+        }
+        if ((tile & 3) == 3) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    acc64[p][c] += static_cast<double>(acc[p][c]);
+                    acc[p][c] = 0.f;
+                }
+        }
+    }
+
+    double v[16];
+    bool ok[16];
+    const int ib = i0 + 4 * ti;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const long col = c0 + 4 * tc + c;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const Out val = static_cast<Out>((acc64[p][c] + static_cast<double>(acc[p][c])) * scaling);
+            ok[4 * c + p] = (col < ncols) && (ib + p < nx);
+            v[4 * c + p] = static_cast<double>(val);
+            if (ok[4 * c + p]) field[col * nx + ib + p] = val;
+        }
+    }
+    block_moments(local_moments(v, ok), part + 3 * blockIdx.x);
+}
+
+// FP64 direct sum on a Cartesian grid (exact path): same tiling, FP64 phase
+// tables, cos(2 pi t) = cospi(2 t) in FP64, FP64 accumulation.
+template <class Out>
+__global__ __launch_bounds__(256) void direct_grid_f64(const double *__restrict__ tx, const double *__restrict__ ty,
+                                                       const double *__restrict__ tz,
+                                                       const double *__restrict__ amp, int np_, int nx, int ny,
+                                                       int nz, int tiles_i, double scaling,
+                                                       Out *__restrict__ field, double *__restrict__ part)
+{
+    constexpr int H = 16;
+    __shared__ double sx[H][kDirB];
+    __shared__ double syz[H][kDirB];
+    __shared__ double sa[H];
+    const int tid = threadIdx.x;
+    const int ti = tid & 15, tc = tid >> 4;
+    const int i0 = (blockIdx.x % tiles_i) * kDirB;
+    const long c0 = long(blockIdx.x / tiles_i) * kDirB;
+    const long ncols = long(ny) * nz;
+    const int lcol = tid & 63, lh = tid >> 6;
+    const long lc = c0 + lcol;
+    const bool lc_ok = lc < ncols;
+    const int lj = lc_ok ? int(lc % ny) : 0, lk = lc_ok ? int(lc / ny) : 0;
+    const int li = i0 + lcol;
+    const bool li_ok = li < nx;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int n0 = 0; n0 < np_; n0 += H) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < H / 4; ++q) {
+            const int h = lh + 4 * q;
+            const long n = n0 + h;
+            sx[h][lcol] = li_ok ? tx[n * nx + li] : 0.0;
+            double t = 0.0;
+            if (lc_ok) {
+                t = ty[n * ny + lj] + tz[n * nz + lk];
+                t -= rint(t);
+            }
+            syz[h][lcol] = t;
+        }
+        if (tid < H) sa[tid] = amp[n0 + tid];
+        __syncthreads();
+        for (int h = 0; h < H; ++h) {
+            const double a = sa[h];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    acc[p][c] = fma(a, cospi(2.0 * (sx[h][4 * ti + p] + syz[h][4 * tc + c])), acc[p][c]);
+        }
+    }
+    double v[16];
+    bool ok[16];
+    const int ib = i0 + 4 * ti;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const long col = c0 + 4 * tc + c;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const Out val = static_cast<Out>(acc[p][c] * scaling);
+            ok[4 * c + p] = (col < ncols) && (ib + p < nx);
+            v[4 * c + p] = static_cast<double>(val);
+            if (ok[4 * c + p]) field[col * nx + ib + p] = val;
+        }
+    }
+    block_moments(local_moments(v, ok), part + 3 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Direct sum on a point list: the phase of every point-harmonic is formed
+// and reduced modulo one turn in FP64 (|w.p| reaches ~1e3 rad, where FP32
+// keeps only ~1e-4 rad); cos and the accumulation run in FP32 (or FP64 for
+// the exact path). 4 points per thread, harmonics staged 64 at a time.
+constexpr int kPtsH = 64;
+constexpr int kPtsP = 4;
+
+template <class Out, bool F64>
+__global__ __launch_bounds__(256) void direct_points(const double *__restrict__ px, const double *__restrict__ py,
+                                                     const double *__restrict__ pz, long npts,
+                                                     const double *__restrict__ harm, int np_, double scaling,
+                                                     Out *__restrict__ field, double *__restrict__ part)
+{
+    __shared__ double sw[kPtsH][4];
+    __shared__ double sa[kPtsH];
+    const int tid = threadIdx.x;
+    const long base = long(blockIdx.x) * 256 * kPtsP;
+    double x[kPtsP], y[kPtsP], z[kPtsP];
+    bool ok[kPtsP];
+#pragma unroll
+    for (int q = 0; q < kPtsP; ++q) {
+        const long p = base + tid + 256 * q;
+        ok[q] = p < npts;
+        x[q] = ok[q] ? px[p] : 0.0;
+        y[q] = ok[q] ? py[p] : 0.0;
+        z[q] = ok[q] ? pz[p] : 0.0;
+    }
+    float acc[kPtsP] = {};
+    double acc64[kPtsP] = {};
+    int tile = 0;
+    for (int n0 = 0; n0 < np_; n0 += kPtsH, ++tile) {
+        __syncthreads();
+        if (tid < kPtsH) {
+            const int n = n0 + tid;
+            sw[tid][0] = harm[n] * kInvTwoPi;
+            sw[tid][1] = harm[np_ + n] * kInvTwoPi;
+            sw[tid][2] = harm[2 * np_ + n] * kInvTwoPi;
+            sw[tid][3] = harm[3 * np_ + n] * kInvTwoPi;
+            sa[tid] = harm[4 * np_ + n];
+        }
+        __syncthreads();
+        for (int h = 0; h < kPtsH; ++h) {
+            const double wx = sw[h][0], wy = sw[h][1], wz = sw[h][2], ph = sw[h][3];
+#pragma unroll
+            for (int q = 0; q < kPtsP; ++q) {
+                double t = fma(wx, x[q], fma(wy, y[q], fma(wz, z[q], ph)));
+                t -= rint(t);
+                if constexpr (F64) {
+                    acc64[q] = fma(sa[h], cospi(2.0 * t), acc64[q]);
+                } else {
+                    acc[q] = fmaf(static_cast<float>(sa[h]), __builtin_amdgcn_cosf(static_cast<float>(t)), acc[q]);
+                }
+            }
+        }
+        if constexpr (!F64) {
+            if ((tile & 1) == 1) {
+#pragma unroll
+                for (int q = 0; q < kPtsP; ++q) {
+                    acc64[q] += static_cast<double>(acc[q]);
+                    acc[q] = 0.f;
+                }
+            }
+        }
+    }
+    double v[kPtsP];
+#pragma unroll
+    for (int q = 0; q < kPtsP; ++q) {
+        const Out val = static_cast<Out>((acc64[q] + static_cast<double>(acc[q])) * scaling);
+        v[q] = static_cast<double>(val);
+        if (ok[q]) field[base + tid + 256 * q] = val;
+    }
+    block_moments(local_moments(v, ok), part + 3 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: rescale (field*s - mean, src/scarflib_srm.f90:897-912) and the
+// mute/taper of every point of interest in order
+// (src/scarflib_common.f90:881-994), at `real` precision without
+// contraction, as the reference computes them on the host.
+template <class Real>
+__device__ inline Real taper_factor(Real val, Real dx2, Real dy2, Real dz2, Real mute, Real taper)
+{
+#pragma clang fp contract(off)
+    const Real d = sqrt(dx2 + dy2 + dz2);
+    const Real pi = static_cast<Real>(3.141592653589793);
+    if (d <= mute) return Real(0);
+    if (d > mute && d <= mute + taper) {
+        const Real t = (d - mute) / taper;
+        return val * (Real(0.5) - Real(0.5) * cos(t * pi));
+    }
+    return val;
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void epilogue_grid(Real *__restrict__ field, int nx, int ny, int nz, int ox,
+                                                     int oy, int oz, double ds_, const double *__restrict__ stat,
+                                                     int rescale, const double *__restrict__ poi, int npoi,
+                                                     double mute_, double taper_)
+{
+#pragma clang fp contract(off)
+    const long n = long(nx) * ny * nz;
+    const Real ds = static_cast<Real>(ds_), mute = static_cast<Real>(mute_), taper = static_cast<Real>(taper_);
+    const Real sc = static_cast<Real>(stat[5]), mu = static_cast<Real>(stat[4]);
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
+        Real v = field[e];
+        if (rescale) v = v * sc - mu;
+        if (npoi > 0) {
+            const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (long(nx) * ny));
+            for (int q = 0; q < npoi; ++q) {
+                Real dx = static_cast<Real>(i + ox) * ds - static_cast<Real>(poi[3 * q]);
+                Real dy = static_cast<Real>(j + oy) * ds - static_cast<Real>(poi[3 * q + 1]);
+                Real dz = static_cast<Real>(k + oz) * ds - static_cast<Real>(poi[3 * q + 2]);
+                v = taper_factor(v, dx * dx, dy * dy, dz * dz, mute, taper);
+            }
+        }
+        field[e] = v;
+    }
+}
+
+template <class Real>
+__global__ __launch_bounds__(256) void epilogue_points(Real *__restrict__ field, long n, const double *__restrict__ px,
+                                                       const double *__restrict__ py, const double *__restrict__ pz,
+                                                       const double *__restrict__ stat, int rescale,
+                                                       const double *__restrict__ poi, int npoi, double mute_,
+                                                       double taper_)
+{
+#pragma clang fp contract(off)
+    const Real mute = static_cast<Real>(mute_), taper = static_cast<Real>(taper_);
+    const Real sc = static_cast<Real>(stat[5]), mu = static_cast<Real>(stat[4]);
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
+        Real v = field[e];
+        if (rescale) v = v * sc - mu;
+        for (int q = 0; q < npoi; ++q) {
+            const Real ax = static_cast<Real>(px[e]) - static_cast<Real>(poi[3 * q]);
+            const Real ay = static_cast<Real>(py[e]) - static_cast<Real>(poi[3 * q + 1]);
+            const Real az = static_cast<Real>(pz[e]) - static_cast<Real>(poi[3 * q + 2]);
+            v = taper_factor(v, ax * ax, ay * ay, az * az, mute, taper);
+        }
+        field[e] = v;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+const char *kernel_name(int k)
+{
+    switch (k) {
+    case kKernelAuto: return "auto";
+    case kKernelDirectF32: return "direct_f32";
+    case kKernelDirectF64: return "direct_f64";
+    case kKernelMfmaBf16x3: return "mfma_bf16x3";
+    case kKernelMfmaF16x2: return "mfma_f16x2";
+    case kKernelMfmaBf16x2: return "mfma_bf16x2";
+    default: return "unknown";
+    }
+}
+
+bool Epilogue::active() const
+{
+    if (rescale == 1) return true;
+    // with mute + taper <= 0 no distance satisfies either branch (defaults -999)
+    return !poi3.empty() && (mute + taper > 0.0 || mute >= 0.0);
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        const hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                              \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
+            return false;                                                                    \
+        }                                                                                    \
+    } while (0)
+
+Device::~Device()
+{
+    if (device_ < 0) return;
+    hipSetDevice(device_);
+    if (stream_) hipStreamSynchronize(stream_);
+    void *bufs[] = {d_harm_, d_tab_, d_pts_, d_part_, d_stat_, d_field_, d_poi_};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (h_stat_) hipHostFree(h_stat_);
+    for (auto &e : ev_)
+        if (e) hipEventDestroy(e);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+bool Device::ensure(std::string &err)
+{
+    if (device_ >= 0) {
+        HIP_TRY(hipSetDevice(device_));
+        return true;
+    }
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) {
+        err = "no HIP device available (the SRM runs only on the GPU; there is no CPU fallback)";
+        return false;
+    }
+    HIP_TRY(hipGetDevice(&device_));
+    HIP_TRY(hipSetDevice(device_));
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto &ev : ev_) HIP_TRY(hipEventCreate(&ev));
+    HIP_TRY(hipMalloc(&d_stat_, 16 * sizeof(double)));
+    HIP_TRY(hipHostMalloc(&h_stat_, 16 * sizeof(double)));
+    return true;
+}
+
+bool Device::reserve(void **ptr, size_t &cap, size_t bytes, std::string &err)
+{
+    if (bytes <= cap && *ptr) return true;
+    if (*ptr) {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        HIP_TRY(hipFree(*ptr));
+        *ptr = nullptr;
+        cap = 0;
+    }
+    HIP_TRY(hipMalloc(ptr, std::max<size_t>(bytes, 256)));
+    cap = bytes;
+    return true;
+}
+
+void *Device::field_buffer(size_t bytes, std::string &err)
+{
+    if (!reserve(&d_field_, field_cap_, bytes, err)) return nullptr;
+    return d_field_;
+}
+
+bool Device::upload_table(const HarmonicTable &t, std::string &err)
+{
+    const int n = t.count();
+    const int np_ = static_cast<int>(round_up(std::max(n, 1), kHarmPad));
+    std::vector<double> rec(size_t(5) * np_, 0.0);
+    for (int l = 0; l < n; ++l) {
+        rec[l] = t.wx[l];
+        rec[np_ + l] = t.wy[l];
+        rec[2 * size_t(np_) + l] = t.wz[l];
+        rec[3 * size_t(np_) + l] = t.phase[l];
+        rec[4 * size_t(np_) + l] = t.amp[l];
+    }
+    void *p = d_harm_;
+    if (!reserve(&p, harm_cap_, rec.size() * sizeof(double), err)) return false;
+    d_harm_ = static_cast<double *>(p);
+    HIP_TRY(hipMemcpyAsync(d_harm_, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    nharm_ = np_;
+    return true;
+}
+
+template <class Real>
+bool Device::upload_points(const Real *x, const Real *y, const Real *z, long n, std::string &err)
+{
+    std::vector<double> pts(size_t(3) * n);
+    for (long i = 0; i < n; ++i) {
+        pts[i] = x[i];
+        pts[n + i] = y[i];
+        pts[2 * n + i] = z ? static_cast<double>(z[i]) : 0.0;
+    }
+    void *p = d_pts_;
+    if (!reserve(&p, pts_cap_, pts.size() * sizeof(double), err)) return false;
+    d_pts_ = static_cast<double *>(p);
+    HIP_TRY(hipMemcpyAsync(d_pts_, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    npts_ = n;
+    return true;
+}
+
+template <class Real>
+bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
+                 double *stats_out, std::string &err)
+{
+    HIP_TRY(hipSetDevice(device_));
+    if (!d_harm_ || nharm_ <= 0) {
+        err = "no harmonic table resident on the device";
+        return false;
+    }
+    const int np_ = nharm_;
+    const long npts = g.points();
+    if (npts < 1) {
+        err = "empty field";
+        return false;
+    }
+    timings_ = Timings{};
+    HIP_TRY(hipEventRecord(ev_[0], stream_));
+
+    int nparts = 0;
+    if (g.structured) {
+        const bool mfma = kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 || kernel == kKernelMfmaBf16x2;
+        if (mfma) {
+            MfmaPlan plan;
+            if (!mfma_plan(g, np_, kernel, plan, err)) return false;
+            void *p = d_tab_;
+            if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;
+            d_tab_ = p;
+            nparts = plan.nparts;
+            void *q = d_part_;
+            if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
+            d_part_ = static_cast<double *>(q);
+            if (!mfma_tables(plan, g, d_harm_, np_, d_tab_, stream_, err)) return false;
+            HIP_TRY(hipEventRecord(ev_[1], stream_));
+            if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err)) return false;
+        } else {
+            const bool f64 = kernel == kKernelDirectF64;
+            const size_t elt = f64 ? sizeof(double) : sizeof(float);
+            const size_t tab_bytes = size_t(np_) * (size_t(g.nx) + g.ny + g.nz) * elt + size_t(np_) * sizeof(float);
+            void *p = d_tab_;
+            if (!reserve(&p, tab_cap_, tab_bytes, err)) return false;
+            d_tab_ = p;
+            const int tiles_i = int((g.nx + kDirB - 1) / kDirB);
+            const long tiles_c = (long(g.ny) * g.nz + kDirB - 1) / kDirB;
+            nparts = int(tiles_i * tiles_c);
+            void *q = d_part_;
+            if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
+            d_part_ = static_cast<double *>(q);
+            const long tab_elems = long(np_) * (g.nx + g.ny + g.nz);
+            const int tblocks = int(std::min<long>((tab_elems + 255) / 256, 65536));
+            if (f64) {
+                double *tx = static_cast<double *>(d_tab_);
+                double *ty = tx + size_t(np_) * g.nx;
+                double *tz = ty + size_t(np_) * g.ny;
+                axis_tables<double><<<tblocks, 256, 0, stream_>>>(d_harm_, np_, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz,
+                                                                  g.ds, tx, ty, tz, nullptr);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(ev_[1], stream_));
+                direct_grid_f64<Real><<<nparts, 256, 0, stream_>>>(tx, ty, tz, d_harm_ + 4 * size_t(np_), np_, g.nx,
+                                                                   g.ny, g.nz, tiles_i, scaling, d_field, d_part_);
+            } else {
+                float *tx = static_cast<float *>(d_tab_);
+                float *ty = tx + size_t(np_) * g.nx;
+                float *tz = ty + size_t(np_) * g.ny;
+                float *amp = tz + size_t(np_) * g.nz;
+                axis_tables<float><<<tblocks, 256, 0, stream_>>>(d_harm_, np_, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz,
+                                                                 g.ds, tx, ty, tz, amp);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(ev_[1], stream_));
+                direct_grid_f32<Real><<<nparts, 256, 0, stream_>>>(tx, ty, tz, amp, np_, g.nx, g.ny, g.nz, tiles_i,
+                                                                   scaling, d_field, d_part_);
+            }
+        }
+        HIP_TRY(hipGetLastError());
+    } else {
+        if (!d_pts_ || npts_ != npts) {
+            err = "point coordinates are not resident on the device";
+            return false;
+        }
+        if (kernel != kKernelDirectF32 && kernel != kKernelDirectF64) {
+            err = "point lists run on the direct kernels only (direct_f32 or direct_f64)";
+            return false;
+        }
+        HIP_TRY(hipEventRecord(ev_[1], stream_));
+        nparts = int((npts + 256 * kPtsP - 1) / (256 * kPtsP));
+        void *q = d_part_;
+        if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
+        d_part_ = static_cast<double *>(q);
+        if (kernel == kKernelDirectF64)
+            direct_points<Real, true><<<nparts, 256, 0, stream_>>>(d_pts_, d_pts_ + npts, d_pts_ + 2 * npts, npts,
+                                                                   d_harm_, np_, scaling, d_field, d_part_);
+        else
+            direct_points<Real, false><<<nparts, 256, 0, stream_>>>(d_pts_, d_pts_ + npts, d_pts_ + 2 * npts, npts,
+                                                                    d_harm_, np_, scaling, d_field, d_part_);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ev_[2], stream_));
+    stats_finalize<Real><<<1, 256, 0, stream_>>>(d_part_, nparts, ep.sigma, d_stat_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev_[3], stream_));
+
+    if (ep.active()) {
+        const int npoi = static_cast<int>(ep.poi3.size() / 3);
+        if (npoi > 0) {
+            void *p = d_poi_;
+            if (!reserve(&p, poi_cap_, ep.poi3.size() * sizeof(double), err)) return false;
+            d_poi_ = static_cast<double *>(p);
+            HIP_TRY(hipMemcpyAsync(d_poi_, ep.poi3.data(), ep.poi3.size() * sizeof(double), hipMemcpyHostToDevice,
+                                   stream_));
+        }
+        const int blocks = int(std::min<long>((npts + 255) / 256, 16384));
+        if (g.structured)
+            epilogue_grid<Real><<<blocks, 256, 0, stream_>>>(d_field, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.ds,
+                                                             d_stat_, ep.rescale == 1, d_poi_, npoi, ep.mute,
+                                                             ep.taper);
+        else
+            epilogue_points<Real><<<blocks, 256, 0, stream_>>>(d_field, npts, d_pts_, d_pts_ + npts,
+                                                               d_pts_ + 2 * npts, d_stat_, ep.rescale == 1, d_poi_,
+                                                               npoi, ep.mute, ep.taper);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ev_[4], stream_));
+    HIP_TRY(hipMemcpyAsync(h_stat_, d_stat_, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+    timings_.tables = ms;
+    hipEventElapsedTime(&ms, ev_[1], ev_[2]);
+    timings_.field = ms;
+    hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+    timings_.stats = ms;
+    hipEventElapsedTime(&ms, ev_[3], ev_[4]);
+    timings_.epilogue = ms;
+    hipEventElapsedTime(&ms, ev_[0], ev_[4]);
+    timings_.total = ms;
+    for (int i = 0; i < 6; ++i) stats_out[i] = h_stat_[i];
+    return true;
+}
+
+template bool Device::upload_points<float>(const float *, const float *, const float *, long, std::string &);
+template bool Device::upload_points<double>(const double *, const double *, const double *, long, std::string &);
+template bool Device::run<float>(const Geometry &, int, double, const Epilogue &, float *, double *, std::string &);
+template bool Device::run<double>(const Geometry &, int, double, const Epilogue &, double *, double *,
+                                  std::string &);
+
+}  // namespace scarf

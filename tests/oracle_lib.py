@@ -1,0 +1,222 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) and of the reference
+QUADPACK harness (oracle/_ref/libref_quadpack.so) for the tests.
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_SO = ORACLE_DIR / "liboracle.so"
+REF_SO = ORACLE_DIR / "_ref" / "libref_quadpack.so"
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nd", C.c_int), ("structured", C.c_int), ("fs", C.c_int * 3), ("fe", C.c_int * 3),
+        ("ds", C.c_double), ("dh", C.c_double), ("acf", C.c_int), ("cl", C.c_double * 3),
+        ("sigma", C.c_double), ("hurst", C.c_double), ("nc", C.c_double * 3), ("fc", C.c_double * 3),
+        ("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double), ("mute", C.c_double),
+        ("taper", C.c_double), ("rescale", C.c_int), ("nharm", C.c_int), ("npoi", C.c_int),
+        ("poi", _DP), ("npts", C.c_int), ("x", _DP), ("y", _DP), ("z", _DP),
+    ]
+
+
+class OrcSetup(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int), ("kmin", C.c_double), ("kmax", C.c_double), ("kc", C.c_double),
+        ("info1", C.c_double), ("info2", C.c_double), ("scaling", C.c_double), ("intg_band", C.c_double),
+        ("intg_all", C.c_double), ("matrix", C.c_double * 9), ("bar", C.c_double * 3), ("nu", C.c_double),
+        ("cnst", C.c_double),
+    ]
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "all"], check=True)
+
+
+class Oracle:
+    def __init__(self):
+        if not ORACLE_SO.exists():
+            build_oracle()
+        L = C.CDLL(str(ORACLE_SO))
+        self.L = L
+        for sfx in ("f", "d"):
+            getattr(L, f"orc_setup_{sfx}").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup)]
+            getattr(L, f"orc_setup_{sfx}").restype = C.c_int
+            getattr(L, f"orc_draw_{sfx}").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup), C.c_int, _DP,
+                                                     C.POINTER(C.c_uint8)]
+            getattr(L, f"orc_draw_{sfx}").restype = C.c_long
+            getattr(L, f"orc_intg_raw_{sfx}").argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                                         C.c_double, _DP, _IP, _IP]
+            getattr(L, f"orc_intg_raw_{sfx}").restype = C.c_double
+            getattr(L, f"orc_srm_{sfx}").argtypes = [C.POINTER(OrcParams), C.c_int, C.c_void_p, _DP]
+            getattr(L, f"orc_srm_{sfx}").restype = C.c_int
+            getattr(L, f"orc_mean_{sfx}").argtypes = [C.c_void_p, C.c_long]
+            getattr(L, f"orc_mean_{sfx}").restype = C.c_double
+            getattr(L, f"orc_variance_{sfx}").argtypes = [C.c_void_p, C.c_long]
+            getattr(L, f"orc_variance_{sfx}").restype = C.c_double
+            getattr(L, f"orc_taper_structured_{sfx}").argtypes = [C.c_void_p, _IP, _IP, C.c_double, _DP,
+                                                                 C.c_double, C.c_double]
+            getattr(L, f"orc_taper_structured_{sfx}").restype = None
+            getattr(L, f"orc_taper_unstructured_{sfx}").argtypes = [C.c_void_p, C.c_long, _DP, _DP, _DP, _DP,
+                                                                   C.c_double, C.c_double]
+            getattr(L, f"orc_taper_unstructured_{sfx}").restype = None
+            getattr(L, f"orc_baseline_structured_{sfx}").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup),
+                                                                    _DP, C.POINTER(C.c_uint8), C.c_int, C.c_int,
+                                                                    C.c_void_p, C.c_int]
+            getattr(L, f"orc_baseline_structured_{sfx}").restype = None
+        L.orc_field_structured.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup), _DP, C.POINTER(C.c_uint8),
+                                           C.c_int, C.c_int, _DP]
+        L.orc_field_structured.restype = None
+        L.orc_field_structured_at.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup), _DP,
+                                              C.POINTER(C.c_uint8), C.c_long, _IP, _IP, _IP, _DP]
+        L.orc_field_structured_at.restype = None
+        L.orc_field_unstructured.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcSetup), _DP, C.POINTER(C.c_uint8),
+                                             _DP]
+        L.orc_field_unstructured.restype = None
+        L.orc_yarn2_stream.argtypes = [C.c_long, C.c_int, C.POINTER(C.c_int32)]
+        L.orc_uniform_stream.argtypes = [C.c_long, C.c_int, _DP]
+        L.orc_powmod.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+        L.orc_powmod.restype = C.c_int64
+        L.orc_parallel_variance.argtypes = [_DP, _DP, C.POINTER(C.c_long), C.c_int, _DP, _DP]
+        L.orc_dqng.restype = C.c_double
+        L.orc_dqagi.restype = C.c_double
+        self._keep = []
+
+    # ------------------------------------------------------------------
+    def params(self, cfg: dict) -> OrcParams:
+        """Resolved parameters (the values the object layer holds) -> OrcParams."""
+        p = OrcParams()
+        p.nd = cfg.get("nd", 3)
+        p.structured = 1 if cfg.get("structured", True) else 0
+        for i in range(3):
+            p.fs[i] = cfg.get("fs", (1, 1, 1))[i]
+            p.fe[i] = cfg.get("fe", (1, 1, 1))[i]
+            p.cl[i] = cfg["cl"][i]
+            p.nc[i] = cfg["nc"][i]
+            p.fc[i] = cfg["fc"][i]
+        p.ds, p.dh, p.acf = cfg["ds"], cfg["dh"], cfg["acf"]
+        p.sigma, p.hurst = cfg["sigma"], cfg.get("hurst", 0.0)
+        p.alpha, p.beta, p.gamma = cfg.get("alpha", 0.0), cfg.get("beta", 0.0), cfg.get("gamma", 0.0)
+        p.mute, p.taper = cfg.get("mute", -999.0), cfg.get("taper", -999.0)
+        p.rescale = cfg.get("rescale", 0)
+        p.nharm = cfg.get("nharm", 10000)
+        poi = cfg.get("poi")
+        if poi is not None:
+            a = np.ascontiguousarray(np.asarray(poi, dtype=np.float64).reshape(-1, order="F"))
+            self._keep.append(a)
+            p.poi = a.ctypes.data_as(_DP)
+            p.npoi = int(np.asarray(poi).shape[1])
+        if not p.structured:
+            xs = [np.ascontiguousarray(np.asarray(cfg[k], dtype=np.float64)) for k in ("x", "y", "z")]
+            self._keep.extend(xs)
+            p.npts = len(xs[0])
+            p.x, p.y, p.z = (a.ctypes.data_as(_DP) for a in xs)
+        return p
+
+    def setup(self, p: OrcParams, flavour: str) -> OrcSetup:
+        s = OrcSetup()
+        rc = getattr(self.L, f"orc_setup_{flavour}")(C.byref(p), C.byref(s))
+        if rc != 0:
+            raise ValueError("oracle setup rejected the parameters")
+        return s
+
+    def draw(self, p: OrcParams, s: OrcSetup, seed: int, flavour: str):
+        harm = np.zeros((p.nharm, 5))
+        used = np.zeros(p.nharm, dtype=np.uint8)
+        n = getattr(self.L, f"orc_draw_{flavour}")(C.byref(p), C.byref(s), seed, harm.ctypes.data_as(_DP),
+                                                   used.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return harm, used, n
+
+    def field_structured(self, p, s, harm, used, k0=None, k1=None):
+        nx, ny, nz = (p.fe[i] - p.fs[i] + 1 for i in range(3))
+        k0 = 0 if k0 is None else k0
+        k1 = nz if k1 is None else k1
+        out = np.zeros(nx * ny * (k1 - k0))
+        self.L.orc_field_structured(C.byref(p), C.byref(s), np.ascontiguousarray(harm).ctypes.data_as(_DP),
+                                    used.ctypes.data_as(C.POINTER(C.c_uint8)), k0, k1, out.ctypes.data_as(_DP))
+        return out.reshape((nx, ny, k1 - k0), order="F")
+
+    def field_at(self, p, s, harm, used, ii, jj, kk):
+        ii, jj, kk = (np.ascontiguousarray(a, dtype=np.int32) for a in (ii, jj, kk))
+        out = np.zeros(len(ii))
+        self.L.orc_field_structured_at(C.byref(p), C.byref(s), np.ascontiguousarray(harm).ctypes.data_as(_DP),
+                                       used.ctypes.data_as(C.POINTER(C.c_uint8)), len(ii), ii.ctypes.data_as(_IP),
+                                       jj.ctypes.data_as(_IP), kk.ctypes.data_as(_IP), out.ctypes.data_as(_DP))
+        return out
+
+    def field_unstructured(self, p, s, harm, used):
+        out = np.zeros(p.npts)
+        self.L.orc_field_unstructured(C.byref(p), C.byref(s), np.ascontiguousarray(harm).ctypes.data_as(_DP),
+                                      used.ctypes.data_as(C.POINTER(C.c_uint8)), out.ctypes.data_as(_DP))
+        return out
+
+    def srm(self, p: OrcParams, seed: int, flavour: str):
+        """Whole pipeline with the FP64 field; returns (field as `real`, stats[8])."""
+        n = (np.prod([p.fe[i] - p.fs[i] + 1 for i in range(3)]) if p.structured else p.npts)
+        field = np.zeros(int(n), dtype=np.float32 if flavour == "f" else np.float64)
+        stats = np.zeros(8)
+        rc = getattr(self.L, f"orc_srm_{flavour}")(C.byref(p), seed, field.ctypes.data_as(C.c_void_p),
+                                                   stats.ctypes.data_as(_DP))
+        if rc != 0:
+            raise ValueError("oracle rejected the parameters")
+        return field, stats
+
+    def intg_raw(self, flavour, acf, ndim, nu, finite, k0=0.0, k1=0.0):
+        e, n, i = C.c_double(), C.c_int(), C.c_int()
+        v = getattr(self.L, f"orc_intg_raw_{flavour}")(acf, ndim, nu, finite, k0, k1, C.byref(e), C.byref(n),
+                                                       C.byref(i))
+        return v, e.value, n.value, i.value
+
+    def uniforms(self, seed: int, n: int):
+        out = np.zeros(n)
+        self.L.orc_uniform_stream(seed, n, out.ctypes.data_as(_DP))
+        return out
+
+    def yarn2(self, seed: int, n: int):
+        out = np.zeros(n, dtype=np.int32)
+        self.L.orc_yarn2_stream(seed, n, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+
+class RefQuadpack:
+    """The reference's own QUADPACK, compiled from src/quadpack.f90 by oracle/Makefile."""
+
+    def __init__(self):
+        L = C.CDLL(str(REF_SO))
+        L.ref_intg.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, _DP, _IP,
+                               _IP]
+        L.ref_intg.restype = C.c_double
+        self.L = L
+
+    def intg(self, acf, ndim, nu, single, finite, k0=0.0, k1=0.0):
+        e, n, i = C.c_double(), C.c_int(), C.c_int()
+        v = self.L.ref_intg(acf, ndim, nu, single, finite, k0, k1, C.byref(e), C.byref(n), C.byref(i))
+        return v, e.value, n.value, i.value
+
+
+def structured_cfg(n=(16, 16, 16), ds=50.0, acf=0, cl=(2000.0, 2000.0, 2000.0), sigma=0.05, hurst=0.25,
+                   nharm=256, dh=None, alpha=0.0, beta=0.0, gamma=0.0, fs=None, poi=None, mute=-999.0,
+                   taper=-999.0, rescale=0, real=np.float32):
+    """Resolved structured configuration with the reference defaults
+    (src/scarflib.f90:132-158): nc = (fs-1)*ds, fc = (fe-1)*ds, dh = ds."""
+    nd = 3 if cl[2] > 0 else 2
+    fs = tuple(fs) if fs is not None else (1, 1, 1)
+    fe = tuple(fs[i] + n[i] - 1 for i in range(3))
+    r = real
+    nc = tuple(float(r(r(fs[i] - 1) * r(ds))) for i in range(3))
+    fc = tuple(float(r(r(fe[i] - 1) * r(ds))) for i in range(3))
+    return dict(nd=nd, structured=True, fs=fs, fe=fe, ds=float(r(ds)), dh=float(r(dh if dh else ds)), acf=acf,
+                cl=tuple(float(r(c)) for c in cl), sigma=float(r(sigma)), hurst=float(r(hurst)), nc=nc, fc=fc,
+                alpha=float(r(alpha)), beta=float(r(beta)), gamma=float(r(gamma)), nharm=nharm, poi=poi,
+                mute=float(r(mute)), taper=float(r(taper)), rescale=rescale)

@@ -1,0 +1,155 @@
+"""GPU parity: the HIP kernels against the FP64 oracle on the same harmonic
+draw (same seed, bit-identical tables, see test_host_parity.py).
+
+Tolerances (stated per test) are in units of the `sigma` parameter; the
+bar of the path is max|field - FP64 field| <= 1e-5 sigma for the FP32 and
+MFMA kernels that meet it, 1e-11 sigma for the FP64 kernel.
+"""
+import numpy as np
+import pytest
+
+from srm_cases import init_product, oracle_field, structured_cfg, unstructured_cfg
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"direct_f32": 1e-5, "mfma_bf16x3": 1e-5, "mfma_f16x2": 1e-5, "mfma_bf16x2": 2e-4, "direct_f64": 1e-11}
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _run(lib, cfg, seed, kernel):
+    init_product(lib, cfg)
+    assert lib.set_kernel(kernel) == 0
+    field, stats = lib.execute(seed)
+    err = lib.last_error()
+    assert err == "", err
+    return field, stats
+
+
+GRID_CASES = [
+    ("vk_iso", dict(n=(40, 36, 20), nharm=384)),
+    ("gauss_aniso", dict(n=(33, 30, 17), acf=1, cl=(300.0, 300.0, 100.0), sigma=0.1, nharm=320)),
+    ("rotated_offset", dict(n=(24, 40, 12), fs=(5, 3, 9), alpha=30.0, beta=-10.0, gamma=5.0, nharm=256)),
+    ("dropped_harmonics", dict(n=(20, 20, 20), hurst=0.1, ds=20.0, dh=10.0, nharm=300)),
+    ("grid_2d", dict(n=(64, 48, 1), cl=(2000.0, 2000.0, 0.0), nharm=256)),
+]
+
+
+@pytest.mark.parametrize("kernel", ["direct_f32", "mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2"])
+@pytest.mark.parametrize("name,kw", GRID_CASES, ids=[c[0] for c in GRID_CASES])
+def test_grid_kernels_match_fp64_oracle(oracle, lib_f, name, kw, kernel):
+    cfg = structured_cfg(real=np.float32, **kw)
+    seed = 1234
+    _say(name, kernel, kw)
+    ref, _ = oracle_field(oracle, cfg, seed, "f")
+    field, stats = _run(lib_f, cfg, seed, kernel)
+    assert field.shape == ref.shape
+    err = np.abs(field.astype(np.float64) - ref).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}", flush=True)
+    assert err <= TOL[kernel]
+
+
+@pytest.mark.parametrize("name,kw", GRID_CASES[:3], ids=[c[0] for c in GRID_CASES[:3]])
+def test_grid_fp64_kernel_double_library(oracle, lib_d, name, kw):
+    cfg = structured_cfg(real=np.float64, **kw)
+    _say(name, "direct_f64 (libscarf3d)")
+    ref, _ = oracle_field(oracle, cfg, 77, "d")
+    field, _ = _run(lib_d, cfg, 77, "direct_f64")
+    err = np.abs(field - ref).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}", flush=True)
+    assert err <= TOL["direct_f64"]
+
+
+@pytest.mark.parametrize("nd", [3, 2])
+@pytest.mark.parametrize("kernel", ["direct_f32", "direct_f64"])
+def test_point_list_matches_fp64_oracle(oracle, lib_f, nd, kernel):
+    cfg = unstructured_cfg(npts=3001, nd=nd, nharm=300)
+    _say("points", nd, kernel)
+    ref, _ = oracle_field(oracle, cfg, 7, "f")
+    field, _ = _run(lib_f, cfg, 7, kernel)
+    err = np.abs(field.astype(np.float64) - ref).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}", flush=True)
+    # the FP64 kernel writes float: rounding of the output dominates
+    assert err <= TOL["direct_f32"] if kernel == "direct_f32" else err <= 1e-6
+
+
+@pytest.mark.parametrize("flavour", ["f", "d"])
+def test_pipeline_stats_rescale_taper(oracle, lib_f, lib_d, flavour):
+    """Whole scarf3d_structured_srm: statistics, rescale (Q2) and sequential
+    mute/taper of two POIs against the oracle pipeline."""
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
+    poi = np.array([[300.0, 400.0, 200.0], [900.0, 500.0, 600.0]]).T  # (nd, npoi)
+    cfg = structured_cfg(n=(32, 24, 20), nharm=256, poi=poi, mute=100.0, taper=250.0, rescale=1, real=real)
+    from oracle_lib import OrcParams  # noqa: F401
+
+    _say("pipeline", flavour)
+    p = oracle.params(cfg)
+    ref_field, ref_stats = oracle.srm(p, 4321, flavour)
+    kernel = "direct_f32" if flavour == "f" else "direct_f64"
+    field, stats = _run(lib, cfg, 4321, kernel)
+    f = field.reshape(-1, order="F").astype(np.float64)
+    r = ref_field.astype(np.float64)
+    # muted points are exactly zero in both
+    assert np.array_equal(f == 0.0, r == 0.0)
+    err = np.abs(f - r).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}", flush=True)
+    assert err <= (2e-5 if flavour == "f" else 1e-9)
+    assert stats[0] == ref_stats[0] and stats[1] == ref_stats[1]
+    assert stats[2] == pytest.approx(ref_stats[2], rel=1e-12)  # sigma after rescale
+    assert stats[3] == 0.0
+
+
+@pytest.mark.parametrize("flavour", ["f", "d"])
+def test_statistics_without_rescale(oracle, lib_f, lib_d, flavour):
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
+    cfg = structured_cfg(n=(48, 40, 24), nharm=256, real=real)
+    p = oracle.params(cfg)
+    _, ref_stats = oracle.srm(p, 99, flavour)
+    field, stats = _run(lib, cfg, 99, "direct_f32" if flavour == "f" else "direct_f64")
+    f = field.reshape(-1).astype(np.float64)
+    # stats(3:4) of the field the library returned, by the reference formulas
+    mean = f.mean()
+    var = ((f - mean) ** 2).sum() / (f.size - 1)
+    assert stats[2] == pytest.approx(np.sqrt(var), rel=2e-6 if flavour == "f" else 1e-12)
+    assert stats[3] == pytest.approx(mean, rel=1e-4, abs=1e-7 * cfg["sigma"])
+    # and against the oracle's statistics of its FP64 field
+    assert stats[2] == pytest.approx(ref_stats[2], rel=1e-5)
+    assert stats[0] == ref_stats[0] and stats[1] == ref_stats[1]
+    assert stats[4] > 0.0 and stats[5] > 0.0  # host and device seconds
+
+
+def test_execute_device_and_resident_run(lib_f):
+    """scarf_execute_device / scarf_srm_draw + scarf_srm_run write the same
+    field as scarf_execute."""
+    import torch
+
+    cfg = structured_cfg(n=(64, 64, 32), nharm=512)
+    field, stats = _run(lib_f, cfg, 5, "direct_f32")
+    d = torch.empty(field.size, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st2 = lib_f.execute_device(5, d.data_ptr())
+    assert np.array_equal(d.cpu().numpy(), field.reshape(-1, order="F"))
+    n, host_s = lib_f.srm_draw(5)
+    assert n == 512 and host_s >= 0
+    d.zero_()
+    torch.cuda.synchronize()
+    st3 = lib_f.srm_run(d.data_ptr())
+    assert np.array_equal(d.cpu().numpy(), field.reshape(-1, order="F"))
+    assert st2[2] == stats[2] and st3[2] == stats[2]
+    t = lib_f.timings()
+    assert t["field"] > 0 and t["total"] >= t["field"]
+
+
+def test_same_seed_same_field_different_seed_differs(lib_f):
+    cfg = structured_cfg(n=(32, 32, 16), nharm=256)
+    a, _ = _run(lib_f, cfg, 11, "mfma_bf16x3")
+    lib_f.finalize()
+    b, _ = _run(lib_f, cfg, 11, "mfma_bf16x3")
+    lib_f.finalize()
+    c, _ = _run(lib_f, cfg, 12, "mfma_bf16x3")
+    assert np.array_equal(a, b)
+    assert not np.allclose(a, c)

@@ -1,0 +1,216 @@
+"""CPU tests of the product library's host side and C ABI (no GPU compute):
+
+- both library flavours load and export every function include/*.h declares;
+- the product's setup (band, QUADPACK normalisation, rotation, bar) and its
+  harmonic draw are bit-identical to the oracle's, for both `real` types;
+- the object layer applies the reference defaults and the option presence
+  rules of src/c_binding.c:89-103 / src/scarflib.f90:132-189, :243-294;
+- error paths report instead of crashing, and execute without a GPU fails
+  loudly (no CPU fallback).
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from srm_cases import init_product, structured_cfg, unstructured_cfg
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        c_part = text.split("#ifdef __cplusplus\n}")[0] if "#ifdef __cplusplus\n}" in text else text
+        for m in re.finditer(r"^\s*(?:void|int|long|const char|double|void \*)\s*\*?\s*(scarf_\w+)\s*\(", c_part, re.M):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("flavour", ["f", "d"])
+def test_library_exports_every_declared_function(flavour):
+    import scarf3d_amd
+
+    lib = scarf3d_amd.load(flavour)
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    for name in sorted(declared):
+        assert hasattr(lib.lib, name), name
+    assert declared == set(scarf3d_amd.api.EXPORTS)
+
+
+SETUP_CASES = [
+    dict(),
+    dict(acf=1, cl=(300.0, 300.0, 100.0), sigma=0.1),
+    dict(alpha=30.0, beta=-10.0, gamma=5.0),
+    dict(cl=(2000.0, 2000.0, 0.0), n=(32, 32, 1)),
+    dict(hurst=0.1, ds=20.0, dh=10.0),
+    dict(n=(20, 30, 10), fs=(4, 2, 7), hurst=0.5, cl=(1500.0, 900.0, 600.0)),
+    dict(acf=1, cl=(150.0, 150.0, 0.0), n=(40, 40, 1), alpha=45.0),
+]
+
+
+@pytest.mark.parametrize("flavour", ["f", "d"])
+@pytest.mark.parametrize("case", range(len(SETUP_CASES)))
+def test_setup_and_draw_bit_identical_to_oracle(oracle, lib_f, lib_d, flavour, case):
+    lib = lib_f if flavour == "f" else lib_d
+    cfg = structured_cfg(nharm=400, real=np.float32 if flavour == "f" else np.float64, **SETUP_CASES[case])
+    p = oracle.params(cfg)
+    s = oracle.setup(p, flavour)
+    init_product(lib, cfg)
+    sv = lib.setup_values()
+    assert sv["ndim"] == s.ndim
+    for key, okey in (("kmin", "kmin"), ("kmax", "kmax"), ("kc", "kc"), ("info1", "info1"), ("info2", "info2"),
+                      ("scaling", "scaling"), ("band", "intg_band"), ("all", "intg_all")):
+        assert sv[key] == getattr(s, okey), key
+    assert np.array_equal(sv["matrix"], np.array(s.matrix[:]))
+    assert np.array_equal(sv["bar"], np.array(s.bar[:]))
+    for seed in (1, 1234, -5):
+        h1, u1, n1 = oracle.draw(p, s, seed, flavour)
+        h2, u2, n2 = lib.draw_harmonics(seed, 400)
+        assert n1 == n2
+        assert np.array_equal(u1, u2)
+        assert np.array_equal(h1, h2)
+
+
+def test_unstructured_setup_bit_identical(oracle, lib_f):
+    cfg = unstructured_cfg(npts=500, nharm=200)
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "f")
+    init_product(lib_f, cfg)
+    sv = lib_f.setup_values()
+    assert sv["kmin"] == s.kmin and sv["scaling"] == s.scaling
+    h1, u1, _ = oracle.draw(p, s, 7, "f")
+    h2, u2, _ = lib_f.draw_harmonics(7, 200)
+    assert np.array_equal(h1, h2) and np.array_equal(u1, u2)
+
+
+def test_structured_defaults(lib_f):
+    """initialize_structured defaults (src/scarflib.f90:132-158): dh = ds,
+    method 0, hurst 0, mute = taper = -999, nc/fc from fs/fe."""
+    lib_f.cart_initialize(3, [2, 3, 4], [11, 13, 15], 25.0, 0, [1000.0, 1000.0, 1000.0], 0.1, None)
+    c = lib_f.config_values()
+    assert c["dh"] == 25.0 and c["method"] == 0 and c["hurst"] == 0
+    assert c["mute"] == -999 and c["taper"] == -999 and c["rescale"] == 0
+    assert (c["nc1"], c["nc2"], c["nc3"]) == (25.0, 50.0, 75.0)
+    assert (c["fc1"], c["fc2"], c["fc3"]) == (250.0, 300.0, 350.0)
+
+
+def test_option_presence_rules(lib_f):
+    """Options count only when method == 1, hurst > 0, dh > 0, mute > 0,
+    taper > 0, rescale == 1, nc and fc both set, angles != 0
+    (src/c_binding.c:89-103)."""
+    o = lib_f.opt(method=2, hurst=-0.3, dh=0.0, mute=0.0, taper=-1.0, rescale=2, pad=3, alpha=0.0)
+    lib_f.cart_initialize(3, [1, 1, 1], [8, 8, 8], 10.0, 0, [100.0, 100.0, 100.0], 0.1, o)
+    c = lib_f.config_values()
+    assert c["method"] == 0 and c["hurst"] == 0 and c["dh"] == 10.0
+    assert c["mute"] == -999 and c["taper"] == -999 and c["rescale"] == 0 and c["pad"] == 0
+    lib_f.finalize()
+    o = lib_f.opt(method=1, hurst=0.3, dh=5.0, mute=20.0, taper=40.0, rescale=1, pad=1, alpha=10.0, beta=-3.0,
+                  gamma=2.0, nc=[-5.0, -6.0, -7.0], fc=[500.0, 600.0, 700.0])
+    lib_f.cart_initialize(3, [1, 1, 1], [8, 8, 8], 10.0, 0, [100.0, 100.0, 100.0], 0.1, o)
+    c = lib_f.config_values()
+    assert c["method"] == 1 and c["hurst"] == pytest.approx(0.3) and c["dh"] == 5.0
+    assert c["mute"] == 20.0 and c["taper"] == 40.0 and c["rescale"] == 1 and c["pad"] == 1
+    assert (c["alpha"], c["beta"], c["gamma"]) == pytest.approx((10.0, -3.0, 2.0))
+    assert (c["nc1"], c["fc3"]) == (-5.0, 700.0)
+    lib_f.finalize()
+    # nc without fc is ignored
+    o = lib_f.opt(nc=[-5.0, -6.0, -7.0])
+    lib_f.cart_initialize(3, [1, 1, 1], [8, 8, 8], 10.0, 0, [100.0, 100.0, 100.0], 0.1, o)
+    assert lib_f.config_values()["nc1"] == 0.0
+
+
+def test_unstructured_defaults(lib_f):
+    """nc/fc default to the coordinate extents, ds = dh; 2D zero z
+    (src/scarflib.f90:236-271)."""
+    x = np.array([3.0, -2.0, 7.0])
+    y = np.array([1.0, 4.0, -6.0])
+    z = np.array([0.5, 9.0, 2.0])
+    lib_f.nocart_initialize(3, x, y, z, 20.0, 1, [100.0, 100.0, 50.0], 0.1, None)
+    c = lib_f.config_values()
+    assert (c["nc1"], c["nc2"], c["nc3"]) == (-2.0, -6.0, 0.5)
+    assert (c["fc1"], c["fc2"], c["fc3"]) == (7.0, 4.0, 9.0)
+    assert c["ds"] == 20.0 and c["structured"] == 0 and c["npts"] == 3
+    lib_f.finalize()
+    lib_f.nocart_initialize(2, x, y, None, 20.0, 1, [100.0, 100.0], 0.1, lib_f.opt(ds=40.0))
+    c = lib_f.config_values()
+    assert (c["nc3"], c["fc3"], c["cl3"]) == (0.0, 0.0, 0.0) and c["ds"] == 40.0
+
+
+def test_initialize_twice_and_execute_before_initialize(lib_f):
+    field = np.zeros(8, dtype=np.float32)
+    stats = np.full(8, 7.0, dtype=np.float32)
+    import ctypes as C
+
+    lib_f.lib.scarf_execute(1, field.ctypes.data_as(C.c_void_p), stats.ctypes.data_as(C.POINTER(C.c_float)))
+    assert "before initialize" in lib_f.last_error()
+    assert np.all(stats == 0)
+    lib_f.cart_initialize(3, [1, 1, 1], [2, 2, 2], 10.0, 0, [100.0, 100.0, 100.0], 0.1, None)
+    lib_f.cart_initialize(3, [1, 1, 1], [4, 4, 4], 10.0, 0, [100.0, 100.0, 100.0], 0.1, None)
+    assert "twice" in lib_f.last_error()
+    assert lib_f.config_values()["fe1"] == 2  # the first object is kept
+    lib_f.finalize()
+    lib_f.cart_initialize(3, [1, 1, 1], [4, 4, 4], 10.0, 0, [100.0, 100.0, 100.0], 0.1, None)
+    assert lib_f.last_error() == "" and lib_f.config_values()["fe1"] == 4
+
+
+@pytest.mark.parametrize("opts,msg", [
+    (dict(), "FIM"),                              # method 0 is FIM
+    (dict(method=1), "hurst"),                    # von Karman needs hurst > 0
+])
+def test_parameter_errors_reported(lib_f, opts, msg):
+    lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 0, [2000.0] * 3, 0.05, lib_f.opt(**opts))
+    field, stats = lib_f.execute(1)
+    assert msg in lib_f.last_error()
+    assert np.all(field == 0) and np.all(stats == 0)
+
+
+def test_user_defined_psd_rejected(lib_f):
+    lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 2, [2000.0] * 3, 0.05, lib_f.opt(method=1, hurst=0.3))
+    lib_f.execute(1)
+    assert "acf" in lib_f.last_error()
+
+
+def test_band_without_power_rejected_instead_of_hanging(lib_f):
+    # Gaussian spectrum on a box much smaller than cl: the band holds ~1e-68
+    # of the spectrum and the reference's rejection loop would never end
+    lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 1, [3000.0, 3000.0, 1000.0], 0.1, lib_f.opt(method=1))
+    lib_f.execute(1)
+    assert "no usable power" in lib_f.last_error()
+
+
+def test_execute_without_gpu_fails_loudly(lib_f):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    cfg = structured_cfg(n=(8, 8, 8), nharm=16)
+    init_product(lib_f, cfg)
+    field, stats = lib_f.execute(3)
+    assert "no HIP device" in lib_f.last_error()
+    assert np.all(field == 0) and np.all(stats == 0)
+
+
+def test_nharm_setting(lib_f):
+    assert lib_f.lib.scarf_get_nharm() == 10000 or lib_f.set_nharm(10000) == 0
+    assert lib_f.set_nharm(0) != 0
+    assert lib_f.set_nharm(2500) == 0 and lib_f.lib.scarf_get_nharm() == 2500
+    lib_f.set_nharm(10000)
+
+
+def test_io_writers(lib_f, tmp_path):
+    """scarf_io_one / scarf_io_slice: raw native-endian real, x fastest
+    (single-writer versions of src/scarflib_common.f90:245-646)."""
+    n = (5, 4, 3)
+    f = np.arange(np.prod(n), dtype=np.float32).reshape(n, order="F")
+    lib_f.io_one(n, f, tmp_path / "vol.bin")
+    back = np.fromfile(tmp_path / "vol.bin", dtype=np.float32).reshape(n, order="F")
+    assert np.array_equal(back, f)
+    for axis, plane, ref in (("x", 2, f[1, :, :]), ("y", 4, f[:, 3, :]), ("z", 1, f[:, :, 0])):
+        lib_f.io_slice(n, axis, plane, f, tmp_path / f"s{axis}.bin")
+        got = np.fromfile(tmp_path / f"s{axis}.bin", dtype=np.float32).reshape(ref.shape, order="F")
+        assert np.array_equal(got, ref), axis
