@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""SRM throughput on one MI355X: point-harmonics per second.
+
+A step is one SRM realization on the device with its harmonic table already
+resident in HBM: per-axis operand tables, the field kernel, the FP64
+statistics pass and the rescale/taper epilogue (scarf_srm_run). Host
+sampling of the harmonics (the reference's draw loop) is timed separately.
+
+Default workload (the configuration the metric is quoted on): structured
+512 x 512 x 512 grid, dh = 20 m, von Karman Hurst 0.1, cl = 5000 m,
+sigma = 0.05, 20000 harmonics, seed 1234. The primary value is the split-bf16
+(3-term, 6-product) MFMA kernel; the FP32 direct-sum kernel is measured on
+the same resident table for the speed-up, and every kernel's error is
+measured over the whole field against the FP64 kernel, itself checked
+against the FP64 CPU oracle at sampled points.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+METRIC = "SRM Gpt·harm/s at 512³×20k: split-BF16 MFMA vs FP32 direct, err≤1e-5σ"
+PEAK_MFMA_TFLOPS = 2500.0  # MI355X dense bf16 / fp16
+PEAK_VALU_TFLOPS = 157.3   # MI355X FP32 vector
+MFMA_PRODUCTS = {"mfma_bf16x3": 6, "mfma_f16x2": 3, "mfma_bf16x2": 3}
+
+WORKLOADS = {
+    "precision_512": dict(n=(512, 512, 512), ds=20.0, acf=0, hurst=0.1, cl=(5000.0,) * 3, sigma=0.05, nharm=20000,
+                          seed=1234),
+    "plain_128": dict(n=(128, 128, 128), ds=50.0, acf=0, hurst=0.25, cl=(2000.0,) * 3, sigma=0.05, nharm=5000,
+                      seed=1234),
+    "large_1024": dict(n=(1024, 1024, 512), ds=20.0, acf=0, hurst=0.2, cl=(6000.0, 4000.0, 1500.0), sigma=0.08,
+                       nharm=50000, seed=1234, rescale=1, npoi=16, mute=20.0, taper=500.0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def init(lib, w):
+    from srm_cases import init_product, structured_cfg
+
+    kw = {k: w[k] for k in ("n", "ds", "acf", "hurst", "cl", "sigma", "nharm") if k in w}
+    extra = {}
+    if w.get("npoi"):
+        rng = np.random.default_rng(16)
+        ext = np.array([(w["n"][i] - 1) * w["ds"] for i in range(3)])
+        extra = dict(poi=(rng.uniform(0, 1, (3, w["npoi"])) * ext[:, None]), mute=w["mute"], taper=w["taper"])
+    cfg = structured_cfg(rescale=w.get("rescale", 0), **kw, **extra)
+    init_product(lib, cfg)
+    return cfg
+
+
+def time_kernel(lib, hip, kernel, d_field, steps, warmup):
+    assert lib.set_kernel(kernel) == 0
+    for _ in range(warmup):
+        lib.srm_run(d_field.ptr)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    field_ms, total_ms = [], []
+    for _ in range(steps):
+        stats = lib.srm_run(d_field.ptr)
+        t = lib.timings()
+        field_ms.append(t["field"])
+        total_ms.append(t["total"])
+    hip.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return dict(ms_per_step=wall * 1e3, field_ms=float(np.mean(field_ms)), device_ms=float(np.mean(total_ms)),
+                stats=stats)
+
+
+def cpu_baseline(oracle, cfg, seed, seconds_target=15.0):
+    """The reference loop (harmonic outer, points inner, float sin/cos) on
+    the host cores, over a bounded z-slab subset of the same workload."""
+    import ctypes as C
+
+    from oracle_lib import _DP
+
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "f")
+    harm, used, _ = oracle.draw(p, s, seed, "f")
+    threads = min(16, len(os.sched_getaffinity(0)))
+    nx, ny = cfg["fe"][0], cfg["fe"][1]
+    nh = int(used.sum())
+
+    def run(nslab, nharm_used):
+        u = used.copy()
+        idx = np.flatnonzero(u)
+        u[idx[nharm_used:]] = 0
+        out = np.zeros(nx * ny * nslab, dtype=np.float32)
+        t0 = time.perf_counter()
+        oracle.L.orc_baseline_structured_f(C.byref(p), C.byref(s), harm.ctypes.data_as(_DP),
+                                           u.ctypes.data_as(C.POINTER(C.c_uint8)), 0, nslab,
+                                           out.ctypes.data_as(C.c_void_p), threads)
+        return time.perf_counter() - t0
+
+    # calibrate on a small slice, then run a sample of about seconds_target
+    probe_h = min(nh, 200)
+    tp = run(threads, probe_h)
+    rate = nx * ny * threads * probe_h / tp
+    nslab = max(threads, int(round(seconds_target * rate / (nx * ny * nh) / threads)) * threads)
+    nslab = min(nslab, cfg["fe"][2])
+    hh = nh if nslab >= threads else probe_h
+    t = run(nslab, hh)
+    wps = nx * ny * nslab * hh / t
+    return dict(value=wps / 1e9, unit="Gpt·harm/s", cores=threads, kind="port",
+                sample=f"{nslab} z-slabs of {nx}x{ny} x {hh} harmonics ({t:.1f} s), reference loop order, FP32 libm "
+                       f"sin/cos, {threads} OpenMP threads each owning z-slabs")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--kernel", default="mfma_bf16x3", choices=["mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2",
+                                                                 "direct_f32"])
+    ap.add_argument("--no-compare", action="store_true", help="skip the direct-sum and FP64 reference runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SRM shards by z-slab ranges with no exchange inside the harmonic "
+              "loop, and multi-GPU runs are not part of this build; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import scarf3d_amd
+    from oracle_lib import Oracle
+    from scarf3d_amd.hip import Hip
+
+    w = WORKLOADS[args.workload]
+    lib = scarf3d_amd.load("f")
+    cfg = init(lib, w)
+    npts = int(np.prod(w["n"]))
+    work = float(npts) * w["nharm"]
+    hip = Hip()
+    d_field = hip.empty(npts, np.float32)
+    n_used, host_s = lib.srm_draw(w["seed"])
+    log(f"[bench] {args.workload}: {w['n']} x {w['nharm']} harmonics ({n_used} kept), host draw {host_s:.2f} s")
+
+    res = {args.kernel: time_kernel(lib, hip, args.kernel, d_field, args.steps, args.warmup)}
+    main_field = d_field.numpy()
+    log(f"[bench] {args.kernel}: {res[args.kernel]['field_ms']:.2f} ms field kernel, "
+        f"{res[args.kernel]['ms_per_step']:.2f} ms/step")
+
+    extra = {}
+    if not args.no_compare:
+        others = [k for k in ("direct_f32", "mfma_f16x2", "mfma_bf16x2") if k != args.kernel]
+        for k in others:
+            steps = max(1, min(3, args.steps)) if k == "direct_f32" else args.steps
+            res[k] = time_kernel(lib, hip, k, d_field, steps, 1)
+            log(f"[bench] {k}: {res[k]['field_ms']:.2f} ms field kernel")
+            res[k]["field"] = d_field.numpy()
+        # FP64 reference over the whole field, on the same resident table
+        t0 = time.perf_counter()
+        d_ref = hip.empty(npts, np.float64)
+        lib.srm_run_fp64(d_ref.ptr)
+        hip.synchronize()
+        ref64 = d_ref.numpy()
+        d_ref.free()
+        log(f"[bench] FP64 reference field in {time.perf_counter() - t0:.1f} s")
+        sigma = cfg["sigma"]
+        res[args.kernel]["field"] = main_field
+        errs = {}
+        for k, r in res.items():
+            dif = r["field"].astype(np.float64) - ref64
+            errs[k] = dict(max_err_sigma=float(np.abs(dif).max() / sigma),
+                           rms_err_sigma=float(np.sqrt(np.mean(dif * dif)) / sigma))
+            del dif
+            log(f"[bench] {k}: max|err| = {errs[k]['max_err_sigma']:.2e} sigma, rms {errs[k]['rms_err_sigma']:.2e}")
+        # the FP64 kernel against the CPU oracle at sampled points (same float-flavour draw)
+        oracle = Oracle()
+        p = oracle.params(cfg)
+        s = oracle.setup(p, "f")
+        harm, used, _ = oracle.draw(p, s, w["seed"], "f")
+        rng = np.random.default_rng(5)
+        nsamp = 256
+        ii, jj, kk = (rng.integers(0, w["n"][a], nsamp) for a in range(3))
+        ref_pts = oracle.field_at(p, s, harm, used, ii, jj, kk)
+        flat = ii + w["n"][0] * (jj + w["n"][1] * kk)
+        main_pts = main_field[flat].astype(np.float64)
+        extra["oracle_points"] = dict(
+            n=nsamp, note="FP64 CPU oracle vs the float-library field at sampled points (same draw)",
+            max_err_sigma=float(np.abs(main_pts - ref_pts).max() / sigma))
+        f64_pts = ref64[flat]
+        extra["fp64_kernel_vs_oracle_max_err_sigma"] = float(np.abs(f64_pts - ref_pts).max() / sigma)
+        log(f"[bench] oracle points: {extra['oracle_points']['max_err_sigma']:.2e} sigma; FP64 kernel vs oracle "
+            f"{extra['fp64_kernel_vs_oracle_max_err_sigma']:.2e} sigma")
+        del ref64
+        for k in res:
+            res[k].pop("field", None)
+            res[k].update(errs.get(k, {}))
+
+    main_r = res[args.kernel]
+    P = MFMA_PRODUCTS.get(args.kernel)
+    if P:
+        flops = work * 4 * P
+        achieved = flops / (main_r["field_ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_MFMA_TFLOPS, 4), "traffic": None,
+                "kernel": f"mfma_grid<{args.kernel}>", "flops_per_point_harmonic": 4 * P,
+                "useful_frac": round(work * 4 / (main_r["field_ms"] * 1e-3) / 1e12 / PEAK_MFMA_TFLOPS, 4)}
+    else:
+        achieved = work * 10 / (main_r["field_ms"] * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_VALU_TFLOPS, 4), "traffic": None, "kernel": "direct_grid_f32",
+                "flops_per_point_harmonic": 10}
+
+    cpu = None
+    if not args.no_cpu:
+        oracle = Oracle()
+        cpu = cpu_baseline(oracle, cfg, w["seed"])
+        log(f"[bench] CPU baseline {cpu['value']:.3f} Gpt.harm/s on {cpu['cores']} threads ({cpu['sample']})")
+
+    value = work / (main_r["ms_per_step"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Gpt·harm/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(main_r["ms_per_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3-split MFMA, fp32 accumulate" if P == 6 else args.kernel,
+        "data": "synthetic (harmonics drawn on the host from the configured spectrum)",
+        "config": {"workload": args.workload, "grid": list(w["n"]), "harmonics": w["nharm"], "dh_m": w["ds"],
+                   "acf": "von Karman" if w["acf"] == 0 else "Gaussian", "hurst": w["hurst"], "cl_m": list(w["cl"]),
+                   "sigma": w["sigma"], "seed": w["seed"], "kernel": args.kernel, "host_draw_s": round(host_s, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items() if kk != "stats"}
+                    for k, r in res.items()},
+    }
+    if "direct_f32" in res and args.kernel != "direct_f32":
+        out["speedup_vs_fp32_direct"] = round(res["direct_f32"]["field_ms"] / main_r["field_ms"], 2)
+        out["kernels"]["direct_f32"]["gpt_harm_per_s"] = round(work / (res["direct_f32"]["field_ms"] * 1e-3) / 1e9, 2)
+    for k, r in out["kernels"].items():
+        r.setdefault("gpt_harm_per_s", round(work / (r["field_ms"] * 1e-3) / 1e9, 2))
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    lib.finalize()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -49,6 +49,12 @@ int scarf_execute_device(const int seed, real *d_field, real stats[]);
 int scarf_srm_draw(const int seed, double *host_seconds);
 int scarf_srm_run(real *d_field, real stats[]);
 
+/* FP64 field of the resident table (FP64 kernel, FP64 output, FP64
+ * epilogue) into d_field (device, doubles), whatever `real` is: the exact
+ * reference against which the faster kernels' errors are measured.
+ * stats_out (may be NULL) = {count, mean, variance, sqrt(variance)}. */
+int scarf_srm_run_fp64(double *d_field, double stats_out[4]);
+
 /* Device milliseconds of the last run: {tables, field, statistics,
  * epilogue, total}; returns the number written (<= n). */
 int scarf_last_timings(double *ms, int n);
@@ -60,6 +66,14 @@ void *scarf_stream(void);
  * out[0..8] = {ndim, kmin, kmax, kc, info1, info2, scaling, intg(kmin,kmax),
  * intg()}, out[9..17] = rotation matrix (column-major), out[18..20] = bar. */
 int scarf_setup_values(double *out, int n);
+
+/* Sets one parameter of the initialized object to `value` as given, with the
+ * semantics of a present Fortran OPTIONAL argument of scarf_initialize
+ * (src/scarflib.f90:163-175, :276-288): unlike struct scarf_opt, a value of
+ * 0 or below counts (e.g. mute = 0). Names: method, hurst, dh, ds, mute,
+ * taper, rescale, pad, alpha, beta, gamma, nc1..nc3, fc1..fc3. Used by the
+ * Fortran module m_scarflib. */
+int scarf_set_option(const char *name, double value);
 
 /* Parameters held after initialize (defaults applied), for checks:
  * {nd, structured, fs[3], fe[3], ds, dh, acf, cl[3], sigma, hurst, nc[3],

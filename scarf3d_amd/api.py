@@ -29,7 +29,7 @@ EXPORTS = (
     "scarf_finalize", "scarf_io_one", "scarf_io_slice",
     "scarf_set_nharm", "scarf_get_nharm", "scarf_set_kernel", "scarf_get_kernel", "scarf_last_error",
     "scarf_execute_device", "scarf_srm_draw", "scarf_srm_run", "scarf_last_timings", "scarf_stream",
-    "scarf_setup_values", "scarf_draw_harmonics", "scarf_get_config",
+    "scarf_setup_values", "scarf_draw_harmonics", "scarf_get_config", "scarf_set_option", "scarf_srm_run_fp64",
 )
 
 
@@ -110,6 +110,10 @@ class Library:
         L.scarf_draw_harmonics.restype = C.c_long
         L.scarf_get_config.argtypes = [P(C.c_double), C.c_int]
         L.scarf_get_config.restype = C.c_int
+        L.scarf_set_option.argtypes = [C.c_char_p, C.c_double]
+        L.scarf_set_option.restype = C.c_int
+        L.scarf_srm_run_fp64.argtypes = [C.c_void_p, P(C.c_double)]
+        L.scarf_srm_run_fp64.restype = C.c_int
 
     def _arr(self, a, dtype=None):
         a = np.ascontiguousarray(a, dtype=dtype or self.dtype)
@@ -239,6 +243,15 @@ class Library:
         if self.lib.scarf_srm_run(C.c_void_p(d_field_ptr), self._rp(stats)) != 0:
             raise RuntimeError(self.last_error())
         return stats
+
+    def srm_run_fp64(self, d_field_ptr: int):
+        st = np.zeros(4)
+        if self.lib.scarf_srm_run_fp64(C.c_void_p(d_field_ptr), st.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise RuntimeError(self.last_error())
+        return st
+
+    def set_option(self, name: str, value: float) -> int:
+        return self.lib.scarf_set_option(name.encode(), float(value))
 
     def execute_device(self, seed: int, d_field_ptr: int):
         stats = np.zeros(8, dtype=self.dtype)

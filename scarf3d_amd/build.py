@@ -65,6 +65,37 @@ def _link(objs: list[str], out: Path, verbose: bool) -> None:
         raise RuntimeError(f"linking {out.name} failed:\n{res.stdout}{res.stderr}")
 
 
+FC = os.environ.get("FC_SCARF", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = False) -> list[Path]:
+    """Fortran module m_scarflib for both library flavours and the test driver
+    (tests/fortran/srm_driver.f90), when amdflang is available."""
+    if not Path(FC).exists():
+        return []
+    outs = []
+    driver = ROOT / "tests" / "fortran" / "srm_driver.f90"
+    for flav, defs, libname in (("f", [], "scarf3df"), ("d", ["-DDOUBLE_PREC"], "scarf3d")):
+        moddir = LIB / f"fortran_{flav}"
+        moddir.mkdir(parents=True, exist_ok=True)
+        obj = moddir / "m_scarflib.o"
+        cmds = [[FC, "-cpp", *defs, "-O2", "-fPIC", "-J", str(moddir), "-c", str(HERE / "fortran" / "m_scarflib.f90"),
+                 "-o", str(obj)]]
+        if driver.exists():
+            exe = LIB / f"srm_driver_{flav}"
+            cmds.append([FC, "-cpp", *defs, "-O2", "-I", str(moddir), str(driver), str(obj), "-L", str(LIB),
+                         f"-l{libname}", f"-Wl,-rpath,{LIB}", "-o", str(exe)])
+            outs.append(exe)
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            res = subprocess.run(cmd, capture_output=True, text=True, cwd=str(moddir))
+            if res.returncode != 0:
+                raise RuntimeError(f"Fortran build failed:\n{res.stdout}{res.stderr}")
+        outs.append(obj)
+    return outs
+
+
 def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
     """Compile every HIP translation unit for gfx950 and link both libraries."""
     BUILD.mkdir(exist_ok=True)
@@ -81,6 +112,7 @@ def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
         _link([*shared, obj], tmp, verbose)
         os.replace(tmp, out)
         outs.append(out)
+    outs.extend(build_fortran(verbose))
     return outs
 
 

@@ -499,6 +499,22 @@ int scarf_srm_run(real *d_field, real stats[])
     return run(d_field, stats);
 }
 
+int scarf_srm_run_fp64(double *d_field, double stats_out[4])
+{
+    State &s = state();
+    if (!d_field) return fail("scarf_srm_run_fp64: d_field is NULL");
+    if (!s.table_ready) return fail("no harmonic table drawn (call scarf_srm_draw first)");
+    const scarf::Geometry g = geometry(s.cfg);
+    const scarf::Epilogue ep = epilogue(s.cfg, s.setup.ndim);
+    double st[6] = {};
+    std::string err;
+    if (!s.device->run<double>(g, scarf::kKernelDirectF64, s.setup.scaling, ep, d_field, st, err)) return fail(err);
+    if (stats_out)
+        for (int i = 0; i < 4; ++i) stats_out[i] = st[i];
+    s.error.clear();
+    return 0;
+}
+
 int scarf_last_timings(double *ms, int n)
 {
     State &s = state();
@@ -529,6 +545,34 @@ int scarf_setup_values(double *out, int n)
     for (int i = 0; i < 9; ++i) v[9 + i] = su.matrix[i];
     for (int i = 0; i < 3; ++i) v[18 + i] = su.bar[i];
     for (int i = 0; i < std::min(n, 21); ++i) out[i] = v[i];
+    return 0;
+}
+
+int scarf_set_option(const char *name, double value)
+{
+    State &s = state();
+    if (!s.initialized) return fail("scarf_set_option: not initialized");
+    if (!name) return fail("scarf_set_option: name is NULL");
+    SrmConfig &c = s.cfg;
+    const std::string n(name);
+    const double v = static_cast<real>(value);
+    if (n == "method") c.method = static_cast<int>(value);
+    else if (n == "hurst") c.hurst = v;
+    else if (n == "dh") c.dh = v;
+    else if (n == "ds") c.ds = v;
+    else if (n == "mute") c.mute = v;
+    else if (n == "taper") c.taper = v;
+    else if (n == "rescale") c.rescale = static_cast<int>(value);
+    else if (n == "pad") c.pad = static_cast<int>(value);
+    else if (n == "alpha") c.alpha = v;
+    else if (n == "beta") c.beta = v;
+    else if (n == "gamma") c.gamma = v;
+    else if (n.size() == 3 && (n.rfind("nc", 0) == 0 || n.rfind("fc", 0) == 0) && n[2] >= '1' && n[2] <= '3')
+        (n[0] == 'n' ? c.nc : c.fc)[n[2] - '1'] = v;
+    else
+        return fail("scarf_set_option: unknown option " + n);
+    s.table_ready = false;
+    s.error.clear();
     return 0;
 }
 
