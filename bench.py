@@ -46,6 +46,98 @@ WORKLOADS = {
 }
 
 
+def _sig(v, digits=4):
+    """v rounded to `digits` significant digits (errors of 1e-6 stay visible)."""
+    return float(f"{v:.{digits}g}")
+
+
+def ensemble_members():
+    """The 64-realization ensemble: uneven grids 96^3 .. 384x384x192, dh 25-100
+    m, von Karman (Hurst 0.05-0.5) and Gaussian spectra, cl 500-8000 m, 2k-16k
+    harmonics, seeds 1..64. Gaussian members keep cl <= span/2 so that their
+    sampling band holds spectral power (with cl far above the box the band sits
+    ~1e-40 below the peak and rejection sampling never ends)."""
+    out = []
+    for r in range(1, 65):
+        nx = 96 + 32 * ((r - 1) % 10)
+        nz = min(nx, 96 + 32 * ((r - 1) % 4))
+        dh = 25.0 * (1 + (r - 1) % 4)
+        acf = (r + 1) % 2
+        hurst = round(0.05 + 0.05 * ((r - 1) % 10), 2)
+        cl = 500.0 * 2 ** ((r - 1) % 5)
+        if acf == 1:
+            cl = min(cl, 0.5 * (min(nx, nz) - 1) * dh)
+        out.append(dict(nd=3, fs=(1, 1, 1), fe=(nx, nx, nz), ds=dh, dh=dh, acf=acf, cl=(cl, cl, cl), sigma=0.05,
+                        hurst=hurst, nharm=2000 * (1 + (r - 1) % 8), seed=r))
+    return out
+
+
+def bench_ensemble(args):
+    import scarf3d_amd
+    from scarf3d_amd.hip import Hip
+
+    lib = scarf3d_amd.load("f")
+    hip = Hip()
+    members = ensemble_members()
+    work = float(sum(np.prod(m["fe"]) * m["nharm"] for m in members))
+    bufs = [hip.empty(int(np.prod(m["fe"])), np.float32) for m in members]
+    ptrs = [b.ptr for b in bufs]
+    t0 = time.perf_counter()
+    stats = lib.ensemble_execute(members, ptrs, args.kernel)
+    log(f"[bench] ensemble first run (draw + graph capture + run) {time.perf_counter() - t0:.2f} s")
+    for _ in range(max(0, args.warmup - 1)):
+        lib.ensemble_execute(members, ptrs, args.kernel)
+    dev_ms, wall = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        stats = lib.ensemble_execute(members, ptrs, args.kernel)
+        wall.append(time.perf_counter() - t1)
+        dev_ms.append(lib.ensemble_ms())
+    host_s = float(np.mean(stats[:, 4].astype(np.float64)))
+    ms = float(np.mean(dev_ms))
+    P = MFMA_PRODUCTS[args.kernel]
+    achieved = work * 4 * P / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "SRM ensemble Gpt·harm/s: 64 uneven realizations, one persistent grouped-GEMM launch in a hipGraph",
+        "value": round(work / (ms * 1e-3) / 1e9, 2), "unit": "Gpt·harm/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.kernel, "data": "synthetic",
+        "config": {"workload": "ensemble_64", "members": len(members), "point_harmonics": work,
+                   "kernel": args.kernel, "host_draw_s_per_step_wall": round(float(np.mean(wall)) - ms * 1e-3, 3),
+                   "host_draw_s_mean_member": round(host_s, 3)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_MFMA_TFLOPS, 4), "traffic": None,
+                     "kernel": f"mfma_ensemble<{args.kernel}>", "flops_per_point_harmonic": 4 * P},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for b in bufs:
+        b.free()
+    return 0
+
+
+def bench_copy_calibration(args):
+    """A device-to-device copy of a known size (1 GiB): the reference point
+    that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    from scarf3d_amd.hip import Hip
+
+    import ctypes as C
+
+    hip = Hip()
+    n = 1 << 28  # floats
+    a = hip.empty(n, np.float32)
+    b = hip.empty(n, np.float32)
+    a.zero()
+    for _ in range(max(1, args.steps)):
+        rc = hip.L.hipMemcpy(C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(4 * n), 3)
+        assert rc == 0
+    hip.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "copies": max(1, args.steps)}), flush=True)
+    a.free()
+    b.free()
+    return 0
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -126,7 +218,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS) + ["ensemble_64", "copy_calib"])
     ap.add_argument("--kernel", default="mfma_bf16x3", choices=["mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2",
                                                                  "direct_f32"])
     ap.add_argument("--no-compare", action="store_true", help="skip the direct-sum and FP64 reference runs")
@@ -136,6 +228,11 @@ def main(argv=None) -> int:
         print("bench.py measures one GPU: the SRM shards by z-slab ranges with no exchange inside the harmonic "
               "loop, and multi-GPU runs are not part of this build; run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.workload == "ensemble_64":
+        return bench_ensemble(args)
+    if args.workload == "copy_calib":
+        return bench_copy_calibration(args)
 
     import scarf3d_amd
     from oracle_lib import Oracle
@@ -236,7 +333,7 @@ def main(argv=None) -> int:
                    "sigma": w["sigma"], "seed": w["seed"], "kernel": args.kernel, "host_draw_s": round(host_s, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items() if kk != "stats"}
+        "kernels": {k: {kk: (_sig(vv) if isinstance(vv, float) else vv) for kk, vv in r.items() if kk != "stats"}
                     for k, r in res.items()},
     }
     if "direct_f32" in res and args.kernel != "direct_f32":

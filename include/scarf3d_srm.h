@@ -55,6 +55,34 @@ int scarf_srm_run(real *d_field, real stats[]);
  * stats_out (may be NULL) = {count, mean, variance, sqrt(variance)}. */
 int scarf_srm_run_fp64(double *d_field, double stats_out[4]);
 
+/* One realization of a Monte Carlo ensemble on a structured grid: the
+ * arguments of scarf_cart_initialize plus the options an ensemble varies
+ * (dh <= 0 means dh = ds; the model box defaults to the grid as in
+ * src/scarflib.f90:157-158; no points of interest). */
+struct scarf_member {
+    int nd;
+    int fs[3], fe[3];
+    real ds, dh;
+    int acf;
+    real cl[3];
+    real sigma, hurst;
+    real alpha, beta, gamma;
+    int rescale;
+    int nharm;
+    int seed;
+};
+
+/* Runs n realizations in one persistent grouped-GEMM launch (MFMA kernel
+ * `kernel`, SCARF_KERNEL_AUTO = mfma_bf16x3) replayed from a hipGraph
+ * captured per ensemble shape. d_fields[m]: device output of member m.
+ * stats: n x 8 values laid out like scarf_execute's. Harmonics are drawn on
+ * the host in parallel, one independent stream per seed. */
+int scarf_ensemble_execute(int n, const struct scarf_member *members, real *const *d_fields, real *stats,
+                           int kernel);
+
+/* Device milliseconds of the last ensemble (graph launch to completion). */
+double scarf_ensemble_last_ms(void);
+
 /* Device milliseconds of the last run: {tables, field, statistics,
  * epilogue, total}; returns the number written (<= n). */
 int scarf_last_timings(double *ms, int n);

@@ -30,6 +30,7 @@ EXPORTS = (
     "scarf_set_nharm", "scarf_get_nharm", "scarf_set_kernel", "scarf_get_kernel", "scarf_last_error",
     "scarf_execute_device", "scarf_srm_draw", "scarf_srm_run", "scarf_last_timings", "scarf_stream",
     "scarf_setup_values", "scarf_draw_harmonics", "scarf_get_config", "scarf_set_option", "scarf_srm_run_fp64",
+    "scarf_ensemble_execute", "scarf_ensemble_last_ms",
 )
 
 
@@ -48,6 +49,17 @@ def _opt_struct(real):
     return ScarfOpt
 
 
+def _member_struct(real):
+    class ScarfMember(C.Structure):
+        # struct scarf_member (include/scarf3d_srm.h)
+        _fields_ = [
+            ("nd", C.c_int), ("fs", C.c_int * 3), ("fe", C.c_int * 3), ("ds", real), ("dh", real), ("acf", C.c_int),
+            ("cl", real * 3), ("sigma", real), ("hurst", real), ("alpha", real), ("beta", real), ("gamma", real),
+            ("rescale", C.c_int), ("nharm", C.c_int), ("seed", C.c_int),
+        ]
+    return ScarfMember
+
+
 class Library:
     """One flavour of the library; flavour 'f' (float real) or 'd' (double real)."""
 
@@ -63,6 +75,7 @@ class Library:
             raise LibraryMissing(f"{self.path} is not built (run: python -m scarf3d_amd.build)")
         self.lib = C.CDLL(str(self.path))
         self.Opt = _opt_struct(self.real)
+        self.Member = _member_struct(self.real)
         self._keep = []  # arrays whose pointers the library borrows
         self._declare()
 
@@ -110,6 +123,10 @@ class Library:
         L.scarf_draw_harmonics.restype = C.c_long
         L.scarf_get_config.argtypes = [P(C.c_double), C.c_int]
         L.scarf_get_config.restype = C.c_int
+        L.scarf_ensemble_execute.argtypes = [C.c_int, P(self.Member), P(C.c_void_p), rp, C.c_int]
+        L.scarf_ensemble_execute.restype = C.c_int
+        L.scarf_ensemble_last_ms.argtypes = []
+        L.scarf_ensemble_last_ms.restype = C.c_double
         L.scarf_set_option.argtypes = [C.c_char_p, C.c_double]
         L.scarf_set_option.restype = C.c_int
         L.scarf_srm_run_fp64.argtypes = [C.c_void_p, P(C.c_double)]
@@ -249,6 +266,32 @@ class Library:
         if self.lib.scarf_srm_run_fp64(C.c_void_p(d_field_ptr), st.ctypes.data_as(C.POINTER(C.c_double))) != 0:
             raise RuntimeError(self.last_error())
         return st
+
+    def ensemble_execute(self, members, d_field_ptrs, kernel="auto"):
+        """members: list of dicts with the scarf_member fields (nd, fs, fe, ds,
+        dh, acf, cl, sigma, hurst, alpha, beta, gamma, rescale, nharm, seed).
+        Returns stats (n x 8)."""
+        n = len(members)
+        arr = (self.Member * n)()
+        for m, d in zip(arr, members):
+            m.nd = d.get("nd", 3)
+            for i in range(3):
+                m.fs[i] = d.get("fs", (1, 1, 1))[i]
+                m.fe[i] = d["fe"][i]
+                m.cl[i] = d["cl"][i]
+            m.ds, m.dh, m.acf = d["ds"], d.get("dh", 0.0), d["acf"]
+            m.sigma, m.hurst = d["sigma"], d.get("hurst", 0.0)
+            m.alpha, m.beta, m.gamma = d.get("alpha", 0.0), d.get("beta", 0.0), d.get("gamma", 0.0)
+            m.rescale, m.nharm, m.seed = d.get("rescale", 0), d["nharm"], d["seed"]
+        ptrs = (C.c_void_p * n)(*[C.c_void_p(p) for p in d_field_ptrs])
+        stats = np.zeros(8 * n, dtype=self.dtype)
+        k = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        if self.lib.scarf_ensemble_execute(n, arr, ptrs, self._rp(stats), k) != 0:
+            raise RuntimeError(self.last_error())
+        return stats.reshape(n, 8)
+
+    def ensemble_ms(self) -> float:
+        return self.lib.scarf_ensemble_last_ms()
 
     def set_option(self, name: str, value: float) -> int:
         return self.lib.scarf_set_option(name.encode(), float(value))

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -27,6 +28,9 @@
 #include "../../include/scarf3d_srm.h"
 #include "srm_device.hpp"
 #include "srm_host.hpp"
+#include "srm_mfma.hpp"
+
+#include <thread>
 
 namespace {
 
@@ -40,16 +44,21 @@ struct State {
     int nharm = 10000;
     std::string error;
     std::unique_ptr<scarf::Device> device;
+    std::unique_ptr<scarf::Ensemble> ensemble;
     // resident realization (scarf_srm_draw -> scarf_srm_run)
     bool table_ready = false;
     scarf::SrmSetup setup;
     double host_seconds = 0;
+    double ensemble_ms = 0;
 };
 
+// Never destroyed: its device buffers, streams and graphs must not be
+// released by static destructors after the HIP runtime (or a profiler hooked
+// into it) has begun its own teardown; the process exit returns them.
 State &state()
 {
-    static State s;
-    return s;
+    static State *s = new State;
+    return *s;
 }
 
 int fail(const std::string &msg)
@@ -498,6 +507,117 @@ int scarf_srm_run(real *d_field, real stats[])
     if (!d_field) return fail("scarf_srm_run: d_field is NULL");
     return run(d_field, stats);
 }
+
+int scarf_ensemble_execute(int n, const struct scarf_member *members, real *const *d_fields, real *stats,
+                           int kernel)
+{
+    State &s = state();
+    if (n < 1 || !members || !d_fields) return fail("scarf_ensemble_execute: empty ensemble or NULL arguments");
+    if (kernel == scarf::kKernelAuto) kernel = scarf::kKernelMfmaBf16x3;
+    if (kernel != scarf::kKernelMfmaBf16x3 && kernel != scarf::kKernelMfmaF16x2 && kernel != scarf::kKernelMfmaBf16x2)
+        return fail("scarf_ensemble_execute: ensembles run on the MFMA kernels");
+    // member configurations with the defaults of initialize_structured
+    std::vector<SrmConfig> cfgs(n);
+    for (int m = 0; m < n; ++m) {
+        const scarf_member &a = members[m];
+        if ((a.nd != 2 && a.nd != 3) || !d_fields[m]) return fail("scarf_ensemble_execute: bad member " + std::to_string(m));
+        SrmConfig &c = cfgs[m];
+        c.nd = a.nd;
+        c.structured = true;
+        c.method = 1;
+        for (int i = 0; i < 3; ++i) {
+            c.fs[i] = i < a.nd ? a.fs[i] : 1;
+            c.fe[i] = i < a.nd ? a.fe[i] : 1;
+            c.cl[i] = i < a.nd ? a.cl[i] : 0;
+            if (c.fe[i] < c.fs[i]) return fail("scarf_ensemble_execute: fe < fs in member " + std::to_string(m));
+            c.nc[i] = static_cast<real>(static_cast<real>(c.fs[i] - 1) * a.ds);
+            c.fc[i] = static_cast<real>(static_cast<real>(c.fe[i] - 1) * a.ds);
+        }
+        c.ds = a.ds;
+        c.dh = a.dh > 0 ? a.dh : a.ds;
+        c.acf = a.acf;
+        c.sigma = a.sigma;
+        c.hurst = a.hurst;
+        c.alpha = a.alpha;
+        c.beta = a.beta;
+        c.gamma = a.gamma;
+        c.rescale = a.rescale;
+        c.nharm = a.nharm > 0 ? a.nharm : s.nharm;
+    }
+    // host draws, one thread per member at a time
+    std::vector<scarf::SrmSetup> setups(n);
+    std::vector<std::vector<double>> records(n);
+    std::vector<int> nps(n, 0);
+    std::vector<double> host_s(n, 0.0);
+    std::vector<std::string> errs(n);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int m = next++; m < n; m = next++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            std::string e;
+            const int rc = kDouble ? scarf::srm_setup<double>(cfgs[m], setups[m], e)
+                                   : scarf::srm_setup<float>(cfgs[m], setups[m], e);
+            if (rc != 0) {
+                errs[m] = e;
+                continue;
+            }
+            const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed)
+                                                  : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed);
+            const scarf::HarmonicTable t = scarf::build_table(setups[m], d);
+            const int np_ = std::max(64, (t.count() + 63) / 64 * 64);
+            std::vector<double> &r = records[m];
+            r.assign(size_t(5) * np_, 0.0);
+            for (int l = 0; l < t.count(); ++l) {
+                r[l] = t.wx[l];
+                r[np_ + l] = t.wy[l];
+                r[2 * size_t(np_) + l] = t.wz[l];
+                r[3 * size_t(np_) + l] = t.phase[l];
+                r[4 * size_t(np_) + l] = t.amp[l];
+            }
+            nps[m] = np_;
+            host_s[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    };
+    const int nthreads = std::max(1, std::min<int>(n, std::min<unsigned>(16, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    for (int m = 0; m < n; ++m)
+        if (!errs[m].empty()) return fail("member " + std::to_string(m) + ": " + errs[m]);
+    if (!ensure_device()) return -1;
+    if (!s.ensemble) s.ensemble = std::make_unique<scarf::Ensemble>();
+    std::vector<scarf::EnsembleMember> em(n);
+    for (int m = 0; m < n; ++m) {
+        em[m].g = geometry(cfgs[m]);
+        em[m].np_ = nps[m];
+        em[m].record = records[m].data();
+        em[m].scaling = setups[m].scaling;
+        em[m].rescale = cfgs[m].rescale;
+        em[m].sigma = cfgs[m].sigma;
+        em[m].d_field = d_fields[m];
+    }
+    std::vector<double> st;
+    double ms = 0;
+    std::string err;
+    if (!s.ensemble->run<real>(s.device->stream(), kernel, em, st, ms, err)) return fail(err);
+    s.ensemble_ms = ms;
+    if (stats) {
+        for (int m = 0; m < n; ++m) {
+            real *o = stats + 8 * size_t(m);
+            for (int i = 0; i < 8; ++i) o[i] = 0;
+            o[0] = static_cast<real>(setups[m].info1);
+            o[1] = static_cast<real>(setups[m].info2);
+            o[2] = cfgs[m].rescale == 1 ? static_cast<real>(cfgs[m].sigma) : static_cast<real>(st[6 * m + 3]);
+            o[3] = cfgs[m].rescale == 1 ? real(0) : static_cast<real>(st[6 * m + 4]);
+            o[4] = static_cast<real>(host_s[m]);
+            o[5] = static_cast<real>(ms * 1e-3);
+        }
+    }
+    s.error.clear();
+    return 0;
+}
+
+double scarf_ensemble_last_ms(void) { return state().ensemble_ms; }
 
 int scarf_srm_run_fp64(double *d_field, double stats_out[4])
 {

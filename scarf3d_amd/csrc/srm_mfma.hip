@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "srm_mfma.hpp"
 #include "srm_stats.hpp"
@@ -154,88 +155,125 @@ __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restr
 }
 
 // ---------------------------------------------------------------------------
-template <int S, class Out>
-__global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf<S>::elt *__restrict__ xt,
-                                                         const float2 *__restrict__ yt,
-                                                         const float2 *__restrict__ zt, int nx, int ny, int nz,
-                                                         int nx_pad, int stages, int tiles_x, int tj_shift, int tk,
-                                                         int tiles_j, double scaling, Out *__restrict__ field,
-                                                         double *__restrict__ part)
+// One stage buffer in LDS; a block double-buffers two of them, declared as
+// two separate __shared__ variables so that the compiler can tell a DMA into
+// one from reads of the other (and does not wait for the DMA before them).
+//   x: P parts x (up to 128 rows x 4 16-byte chunks), chunks XOR-swizzled
+//   y: 16 harmonics x tj values of Y (complex), row stride tj
+//   z: 16 harmonics x tk values of Z (complex), row stride tk
+template <int P>
+struct StageLds {
+    uint4 x[P][kTileX * 4];
+    float2 y[kStageH * kTileC];
+    float2 z[kStageH * 8];
+};
+
+__device__ inline void lds_dma16(const void *src, void *lds_wave_base)
+{
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)lds_wave_base, 16, 0, 0);
+}
+__device__ inline void lds_dma4(const void *src, void *lds_wave_base)
+{
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)lds_wave_base, 4, 0, 0);
+}
+
+// One output tile: 256 grid columns (tj values of j by tk values of k,
+// starting at j0, k0) by TX = 32*NBX x-points starting at x0, over all
+// `stages` stages of 16 harmonics. Every thread of the block calls this with
+// the same arguments.
+//
+// Stage data moves global -> LDS with the LDS-DMA loads of gfx950
+// (global_load_lds: each lane of a wave writes 4 or 16 bytes at
+// M0 + lane * size), issued for stage s+1 before the MFMAs of stage s and
+// waited for once, right before the stage barrier: no registers hold
+// prefetched data. Reads past the ragged grid edges are clamped to valid
+// rows; those columns are never stored.
+template <int S, int NBX, class Out>
+__device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__restrict__ xt,
+                                          const float2 *__restrict__ yt, const float2 *__restrict__ zt, int nx,
+                                          int ny, int nz, int nx_pad, int stages, int tj_shift, int tk, int x0,
+                                          int j0, int k0, double scaling, Out *__restrict__ field,
+                                          double *__restrict__ part_out, StageLds<SchemeOf<S>::parts> &lds0,
+                                          StageLds<SchemeOf<S>::parts> &lds1)
 {
     using Sch = SchemeOf<S>;
     using E = typename Sch::elt;
     using V = typename Sch::vec;
     constexpr int P = Sch::parts;
     constexpr int A = Sch::accs;
-
-    __shared__ uint4 sx[2][P][kTileX * 4];         // 16-byte chunks, swizzled
-    __shared__ float2 syz[2][kStageH][kTileC];     // generated complex operand
+    constexpr int TX = 32 * NBX;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long b = blockIdx.x;
-    const int xt_idx = int(b % tiles_x);
-    const long ct = b / tiles_x;
-    const int j0 = int(ct % tiles_j) << tj_shift;
-    const int k0 = int(ct / tiles_j) * tk;
-    const int x0 = xt_idx * kTileX;
-    const int tj_mask = (1 << tj_shift) - 1;
+    const int tj = 1 << tj_shift;
+    const int tj_mask = tj - 1;
 
-    // loader roles
-    const int lcol = tid & (kTileC - 1), lh0 = tid >> 8;  // column, first harmonic row (0 or 1)
-    const int lj = j0 + (lcol & tj_mask), lk = k0 + (lcol >> tj_shift);
-    const bool lok = lj < ny && lk < nz;
-    const int xrow = tid >> 2, xq = tid & 3;  // x-table chunk copied by this thread
-    const uint4 *xsrc = reinterpret_cast<const uint4 *>(xt);
+    // x-operand chunk of this lane: LDS slot tid, global chunk swizzled back
+    const bool xload = tid < TX * 4;
+    const int xr = tid >> 2;
+    const int xq = (tid & 3) ^ ((xr >> 1) & 3);
+    const uint4 *xsrc = reinterpret_cast<const uint4 *>(xt) + (size_t(x0 + xr) * 4 + xq);
+    const size_t xpart_stride = size_t(nx_pad) * 4;  // uint4 per part of a stage
+    // y / z floats of this lane: index i = q * 512 + tid in the stage tile
+    const int ny_floats = 32 * tj, nz_floats = 32 * tk;
+    const float *yf = reinterpret_cast<const float *>(yt);
+    const float *zf = reinterpret_cast<const float *>(zt);
 
-    uint4 xreg[P];
-    float2 yreg[kStageH / 2], zreg[kStageH / 2];
-    auto issue_loads = [&](int s) {
+    auto issue_loads = [&](int s, StageLds<P> &dst) {
+        if (xload) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) xreg[p] = xsrc[((size_t(s) * P + p) * nx_pad + x0) * 4 + tid];
-#pragma unroll
-        for (int q = 0; q < kStageH / 2; ++q) {
-            const size_t n = size_t(s) * kStageH + lh0 + 2 * q;
-            yreg[q] = lok ? yt[n * ny + lj] : make_float2(0.f, 0.f);
-            zreg[q] = lok ? zt[n * nz + lk] : make_float2(0.f, 0.f);
+            for (int p = 0; p < P; ++p)
+                lds_dma16(xsrc + (size_t(s) * P + p) * xpart_stride, &dst.x[p][wave * 64]);
+        }
+        for (int i0 = 0; i0 < ny_floats; i0 += kThreads) {
+            const int i = i0 + tid;  // ny_floats is a multiple of 512
+            const int h = i / (2 * tj), e = i - h * 2 * tj;
+            const int j = min(j0 + (e >> 1), ny - 1);
+            lds_dma4(yf + (size_t(s * kStageH + h) * ny + j) * 2 + (e & 1),
+                     reinterpret_cast<float *>(dst.y) + i0 + wave * 64);
+        }
+        if (tid < nz_floats) {
+            const int h = tid / (2 * tk), e = tid - h * 2 * tk;
+            const int k = min(k0 + (e >> 1), nz - 1);
+            lds_dma4(zf + (size_t(s * kStageH + h) * nz + k) * 2 + (e & 1),
+                     reinterpret_cast<float *>(dst.z) + wave * 64);
         }
     };
-    auto commit = [&](int buf) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) sx[buf][p][chunk_slot(xrow, xq)] = xreg[p];
-#pragma unroll
-        for (int q = 0; q < kStageH / 2; ++q) {
-            const float2 y = yreg[q], z = zreg[q];
-            syz[buf][lh0 + 2 * q][lcol] = make_float2(z.x * y.x - z.y * y.y, z.x * y.y + z.y * y.x);
-        }
-    };
 
-    f32x16 acc[A][4], main_acc[4];
+    f32x16 acc[A][NBX], main_acc[NBX];
 #pragma unroll
-    for (int bx = 0; bx < 4; ++bx) {
+    for (int bx = 0; bx < NBX; ++bx) {
 #pragma unroll
         for (int a = 0; a < A; ++a) acc[a][bx] = f32x16{};
         main_acc[bx] = f32x16{};
     }
 
-    issue_loads(0);
-    commit(0);
+    __syncthreads();  // the LDS buffers may still be read by the previous tile
+    issue_loads(0, lds0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
-    const int crow = wave * 32 + (lane & 31);  // generated-operand row of this lane
+    const int crow = wave * 32 + (lane & 31);  // generated-operand row (grid column) of this lane
+    const int cj = crow & tj_mask, ck = crow >> tj_shift;
     const int khalf = lane >> 5;
-    for (int s = 0; s < stages; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < stages) issue_loads(s + 1);
+    // stage s from `cur` while stage s+1 streams into `nxt`
+    auto stage = [&](int s, StageLds<P> &cur, StageLds<P> &nxt) __attribute__((always_inline)) {
+        if (s + 1 < stages) issue_loads(s + 1, nxt);
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-            // generated operand: 4 harmonics x (re, im) of this lane's column
+            // generated operand: Z*Y for 4 harmonics x (re, im) of this lane's column
             E gp[P][8];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const float2 v = syz[buf][kk * 8 + khalf * 4 + t][crow];
+                const int h = kk * 8 + khalf * 4 + t;
+                const float2 y = cur.y[h * tj + cj];
+                const float2 z = cur.z[h * tk + ck];
+                const float re = z.x * y.x - z.y * y.y;
+                const float im = z.x * y.y + z.y * y.x;
                 E pr[P], pi[P];
-                split<S>(v.x, pr);
-                split<S>(v.y, pi);
+                split<S>(re, pr);
+                split<S>(im, pi);
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
                     gp[p][2 * t] = pr[p];
@@ -248,12 +286,12 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
 #pragma unroll
                 for (int e = 0; e < 8; ++e) g[p][e] = gp[p][e];
 #pragma unroll
-            for (int bx = 0; bx < 4; ++bx) {
-                const int xr = bx * 32 + (lane & 31);
-                const int slot = chunk_slot(xr, kk * 2 + khalf);
+            for (int bx = 0; bx < NBX; ++bx) {
+                const int row = bx * 32 + (lane & 31);
+                const int slot = chunk_slot(row, kk * 2 + khalf);
                 V xv[P];
 #pragma unroll
-                for (int p = 0; p < P; ++p) xv[p] = __builtin_bit_cast(V, sx[buf][p][slot]);
+                for (int p = 0; p < P; ++p) xv[p] = __builtin_bit_cast(V, cur.x[p][slot]);
                 if constexpr (S == kBf16x3) {
                     acc[0][bx] = mma(g[1], xv[1], acc[0][bx]);
                     acc[0][bx] = mma(g[0], xv[2], acc[0][bx]);
@@ -272,10 +310,9 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
                 }
             }
         }
-        if (s + 1 < stages) commit(buf ^ 1);
         if ((s % kPromote) == kPromote - 1 || s + 1 == stages) {
 #pragma unroll
-            for (int bx = 0; bx < 4; ++bx) {
+            for (int bx = 0; bx < NBX; ++bx) {
                 if constexpr (A == 2) {
                     main_acc[bx] += acc[0][bx] + acc[1][bx] * (1.f / kLoScale);
                 } else {
@@ -285,14 +322,20 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
                 for (int a = 0; a < A; ++a) acc[a][bx] = f32x16{};
             }
         }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // stage s+1 has landed in LDS
         __syncthreads();
+    };
+    for (int s = 0; s < stages; s += 2) {
+        stage(s, lds0, lds1);
+        if (s + 1 == stages) break;
+        stage(s + 1, lds1, lds0);
     }
 
     // epilogue: scale, round to the field type, store, statistics
-    double vals[64];
-    bool ok[64];
+    double vals[16 * NBX];
+    bool ok[16 * NBX];
 #pragma unroll
-    for (int bx = 0; bx < 4; ++bx) {
+    for (int bx = 0; bx < NBX; ++bx) {
         const int x = x0 + bx * 32 + (lane & 31);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -306,7 +349,131 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
             if (inside) field[size_t(x) + size_t(nx) * (size_t(j) + size_t(ny) * k)] = val;
         }
     }
-    block_moments(local_moments(vals, ok), part + 3 * b);
+    block_moments(local_moments(vals, ok), part_out);
+}
+
+template <int S, int NBX, class Out>
+__global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf<S>::elt *__restrict__ xt,
+                                                         const float2 *__restrict__ yt,
+                                                         const float2 *__restrict__ zt, int nx, int ny, int nz,
+                                                         int nx_pad, int stages, int tiles_x, int tj_shift, int tk,
+                                                         int tiles_j, double scaling, Out *__restrict__ field,
+                                                         double *__restrict__ part)
+{
+    __shared__ StageLds<SchemeOf<S>::parts> lds0, lds1;
+    const long b = blockIdx.x;
+    const int x0 = int(b % tiles_x) * 32 * NBX;
+    const long ct = b / tiles_x;
+    const int j0 = int(ct % tiles_j) << tj_shift;
+    const int k0 = int(ct / tiles_j) * tk;
+    gemm_tile<S, NBX, Out>(xt, yt, zt, nx, ny, nz, nx_pad, stages, tj_shift, tk, x0, j0, k0, scaling, field,
+                           part + 3 * b, lds0, lds1);
+}
+
+// Ensemble: one persistent launch over the tiles of many realizations.
+struct MemberDesc {
+    const void *xt;
+    const float2 *yt, *zt;
+    void *field;
+    int nx, ny, nz, nx_pad, stages, tiles_x, tiles_j, tiles_k, tj_shift, tk, nbx;
+    int part_base;  // first statistics triple of this member
+    double scaling, sigma;
+};
+
+__global__ void reset_counter(int *c) { c[0] = 0; }
+
+// per-member merge of the tile triples, then the `real` roundings of
+// src/scarflib_srm.f90:880-899 (one block per member)
+template <class Real>
+__global__ __launch_bounds__(256) void ensemble_finalize(const MemberDesc *__restrict__ members,
+                                                         const double *__restrict__ part, double *__restrict__ stat)
+{
+    __shared__ double sm[3][256];
+    const MemberDesc m = members[blockIdx.x];
+    const int ntiles = m.tiles_x * m.tiles_j * m.tiles_k;
+    Moments acc{0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < ntiles; t += 256) {
+        const double *q = part + 3 * (size_t(m.part_base) + t);
+        merge_moments(acc, Moments{q[0], q[1], q[2]});
+    }
+    sm[0][threadIdx.x] = acc.n;
+    sm[1][threadIdx.x] = acc.mean;
+    sm[2][threadIdx.x] = acc.m2;
+    __syncthreads();
+    for (int stride = 128; stride >= 1; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            Moments a{sm[0][threadIdx.x], sm[1][threadIdx.x], sm[2][threadIdx.x]};
+            merge_moments(a, Moments{sm[0][threadIdx.x + stride], sm[1][threadIdx.x + stride],
+                                     sm[2][threadIdx.x + stride]});
+            sm[0][threadIdx.x] = a.n;
+            sm[1][threadIdx.x] = a.mean;
+            sm[2][threadIdx.x] = a.m2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = sm[0][0], mean = sm[1][0], m2 = sm[2][0];
+        const double var = n > 1.0 ? m2 / (n - 1.0) : 0.0;
+        const Real info3 = sqrt(static_cast<Real>(var));
+        double *o = stat + 6 * blockIdx.x;
+        o[0] = n;
+        o[1] = mean;
+        o[2] = var;
+        o[3] = info3;
+        o[4] = static_cast<Real>(mean);
+        o[5] = static_cast<Real>(m.sigma) / info3;
+    }
+}
+
+// rescale of one member: field * (sigma / sigma_d) - mean (Q2), no contraction
+template <class Real>
+__global__ __launch_bounds__(256) void ensemble_rescale(Real *__restrict__ field, long n,
+                                                        const double *__restrict__ stat)
+{
+#pragma clang fp contract(off)
+    const Real sc = static_cast<Real>(stat[5]), mu = static_cast<Real>(stat[4]);
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x)
+        field[e] = field[e] * sc - mu;
+}
+
+template <int S, class Out>
+__global__ __launch_bounds__(kThreads, 1) void mfma_ensemble(const MemberDesc *__restrict__ members,
+                                                             const int2 *__restrict__ items, int nitems,
+                                                             int *__restrict__ counter, double *__restrict__ part)
+{
+    using E = typename SchemeOf<S>::elt;
+    __shared__ StageLds<SchemeOf<S>::parts> lds0, lds1;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) break;  // every wave of the block sees the same value and leaves together
+        const int2 item = items[it];  // (member, tile within the member, x fastest)
+        const MemberDesc m = members[item.x];
+        const int x0 = (item.y % m.tiles_x) * 32 * m.nbx;
+        const int ct = item.y / m.tiles_x;
+        const int j0 = (ct % m.tiles_j) << m.tj_shift;
+        const int k0 = (ct / m.tiles_j) * m.tk;
+        const E *xt = static_cast<const E *>(m.xt);
+        Out *field = static_cast<Out *>(m.field);
+        double *po = part + 3 * (size_t(m.part_base) + item.y);
+        switch (m.nbx) {
+        case 2:
+            gemm_tile<S, 2, Out>(xt, m.yt, m.zt, m.nx, m.ny, m.nz, m.nx_pad, m.stages, m.tj_shift, m.tk, x0, j0,
+                                 k0, m.scaling, field, po, lds0, lds1);
+            break;
+        case 3:
+            gemm_tile<S, 3, Out>(xt, m.yt, m.zt, m.nx, m.ny, m.nz, m.nx_pad, m.stages, m.tj_shift, m.tk, x0, j0,
+                                 k0, m.scaling, field, po, lds0, lds1);
+            break;
+        default:
+            gemm_tile<S, 4, Out>(xt, m.yt, m.zt, m.nx, m.ny, m.nz, m.nx_pad, m.stages, m.tj_shift, m.tk, x0, j0,
+                                 k0, m.scaling, field, po, lds0, lds1);
+            break;
+        }
+    }
 }
 
 template <int S>
@@ -327,7 +494,7 @@ int scheme_of(int kernel)
 
 }  // namespace
 
-bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err)
+bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx)
 {
     const int s = scheme_of(kernel);
     if (s < 0) {
@@ -343,8 +510,22 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
     plan.parts = (s == kBf16x3) ? 3 : 2;
     plan.np_ = np_;
     plan.stages = np_ / kStageH;
-    plan.tiles_x = (g.nx + kTileX - 1) / kTileX;
-    plan.nx_pad = plan.tiles_x * kTileX;
+    // x tile of 64, 96 or 128 points: the fewest padded x, ties to the wider tile
+    if (nbx < 2 || nbx > 4) {
+        nbx = 4;
+        long best = -1;
+        for (int c = 4; c >= 2; --c) {
+            const long waste = long((g.nx + 32 * c - 1) / (32 * c)) * 32 * c - g.nx;
+            if (best < 0 || waste < best) {
+                best = waste;
+                nbx = c;
+            }
+        }
+    }
+    plan.nbx = nbx;
+    const int tx = 32 * nbx;
+    plan.tiles_x = (g.nx + tx - 1) / tx;
+    plan.nx_pad = plan.tiles_x * tx;
     // column tile tj (j) x tk (k) = 256 columns: the shape wasting the fewest
     // padded columns, ties going to more k (fewer distinct Y rows per stage)
     long best_waste = -1;
@@ -357,6 +538,8 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
             plan.tk = tk;
         }
     }
+    plan.tj_shift = 0;
+    while ((1 << plan.tj_shift) < plan.tj) ++plan.tj_shift;
     plan.tiles_j = (g.ny + plan.tj - 1) / plan.tj;
     plan.tiles_k = (g.nz + plan.tk - 1) / plan.tk;
     plan.blocks = long(plan.tiles_x) * plan.tiles_j * plan.tiles_k;
@@ -369,7 +552,7 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
                                   : x_table_bytes<kBf16x2>(plan.nx_pad, np_);
     plan.y_bytes = size_t(np_) * g.ny * sizeof(float2);
     plan.z_bytes = size_t(np_) * g.nz * sizeof(float2);
-    plan.table_bytes = plan.x_bytes + plan.y_bytes + plan.z_bytes + 256;
+    plan.table_bytes = (plan.x_bytes + plan.y_bytes + plan.z_bytes + 255) / 256 * 256;
     return true;
 }
 
@@ -406,35 +589,38 @@ bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, 
     return true;
 }
 
+namespace {
+
+template <int S, class Real>
+void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, double scaling, Real *d_field,
+                 double *d_part, hipStream_t stream)
+{
+    using E = typename SchemeOf<S>::elt;
+    const E *xt = reinterpret_cast<const E *>(base);
+    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes);
+    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes);
+    const dim3 grid(static_cast<unsigned>(plan.blocks));
+#define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x, plan.tj_shift, plan.tk, \
+                        plan.tiles_j, scaling, d_field, d_part
+    switch (plan.nbx) {
+    case 2: mfma_grid<S, 2, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+    case 3: mfma_grid<S, 3, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+    default: mfma_grid<S, 4, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+    }
+#undef SCARF_GRID_ARGS
+}
+
+}  // namespace
+
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
                 double *d_part, hipStream_t stream, std::string &err)
 {
     const char *base = static_cast<const char *>(d_tab);
-    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes);
-    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes);
-    int tj_shift = 0;
-    while ((1 << tj_shift) < plan.tj) ++tj_shift;
-    const dim3 grid(static_cast<unsigned>(plan.blocks));
     switch (scheme_of(plan.kernel)) {
-    case kBf16x3:
-        mfma_grid<kBf16x3, Real><<<grid, kThreads, 0, stream>>>(reinterpret_cast<const __bf16 *>(base), yt, zt, g.nx,
-                                                                g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x,
-                                                                tj_shift, plan.tk, plan.tiles_j, scaling, d_field,
-                                                                d_part);
-        break;
-    case kBf16x2:
-        mfma_grid<kBf16x2, Real><<<grid, kThreads, 0, stream>>>(reinterpret_cast<const __bf16 *>(base), yt, zt, g.nx,
-                                                                g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x,
-                                                                tj_shift, plan.tk, plan.tiles_j, scaling, d_field,
-                                                                d_part);
-        break;
-    default:
-        mfma_grid<kF16x2s, Real><<<grid, kThreads, 0, stream>>>(reinterpret_cast<const _Float16 *>(base), yt, zt,
-                                                                g.nx, g.ny, g.nz, plan.nx_pad, plan.stages,
-                                                                plan.tiles_x, tj_shift, plan.tk, plan.tiles_j,
-                                                                scaling, d_field, d_part);
-        break;
+    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
+    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
+    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -446,6 +632,207 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
 
 template bool mfma_field<float>(const MfmaPlan &, const Geometry &, const void *, double, float *, double *,
                                 hipStream_t, std::string &);
+
+// ---------------------------------------------------------------------------
+Ensemble::~Ensemble() { release(); }
+
+void Ensemble::release()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    void *bufs[] = {d_rec_, d_tab_, d_part_, d_desc_, d_items_, d_counter_, d_stat_};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    d_rec_ = d_tab_ = d_part_ = d_desc_ = d_items_ = d_counter_ = d_stat_ = nullptr;
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    ev_[0] = ev_[1] = nullptr;
+    key_ = 0;
+}
+
+#define ENS_TRY(expr)                                                                \
+    do {                                                                             \
+        const hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                      \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return false;                                                            \
+        }                                                                            \
+    } while (0)
+
+template <class Real>
+bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMember> &members,
+                   std::vector<double> &stats, double &device_ms, std::string &err)
+{
+    const int s = scheme_of(kernel);
+    if (s < 0) {
+        err = "ensembles run on the MFMA kernels (mfma_bf16x3, mfma_f16x2, mfma_bf16x2)";
+        return false;
+    }
+    const int n = static_cast<int>(members.size());
+    if (n < 1) {
+        err = "empty ensemble";
+        return false;
+    }
+    // plans, offsets and the shape key
+    std::vector<MfmaPlan> plans(n);
+    std::vector<size_t> rec_off(n), tab_off(n);
+    std::vector<int> part_base(n);
+    size_t rec_total = 0, tab_total = 0;
+    int parts_total = 0;
+    size_t key = 1469598103934665603ull;
+    auto mix = [&](size_t v) { key = (key ^ v) * 1099511628211ull; };
+    mix(size_t(kernel));
+    mix(sizeof(Real));
+    for (int m = 0; m < n; ++m) {
+        const EnsembleMember &e = members[m];
+        if (!e.g.structured) {
+            err = "ensemble members must be structured grids";
+            return false;
+        }
+        if (!mfma_plan(e.g, e.np_, kernel, plans[m], err, e.nbx)) return false;
+        rec_off[m] = rec_total;
+        rec_total += size_t(5) * e.np_;
+        tab_off[m] = tab_total;
+        tab_total += plans[m].table_bytes;
+        part_base[m] = parts_total;
+        parts_total += plans[m].nparts;
+        const size_t vals[] = {size_t(e.g.nx), size_t(e.g.ny), size_t(e.g.nz), size_t(e.g.ox), size_t(e.g.oy),
+                               size_t(e.g.oz), size_t(e.np_), size_t(plans[m].nbx), size_t(e.rescale),
+                               reinterpret_cast<size_t>(e.d_field)};
+        for (size_t v : vals) mix(v);
+        double ds = e.g.ds, sg = e.sigma;
+        size_t bits;
+        std::memcpy(&bits, &ds, sizeof bits);
+        mix(bits);
+        std::memcpy(&bits, &sg, sizeof bits);
+        mix(bits);
+    }
+
+    // work items, longest members first (all tiles of a member cost the same)
+    std::vector<int> order(n);
+    for (int m = 0; m < n; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return long(plans[a].stages) * plans[a].nbx > long(plans[b].stages) * plans[b].nbx;
+    });
+    std::vector<int2> items;
+    for (int m : order)
+        for (int t = 0; t < plans[m].nparts; ++t) items.push_back(make_int2(m, t));
+
+    if (key != key_ || !exec_) {
+        release();
+        ENS_TRY(hipEventCreate(&ev_[0]));
+        ENS_TRY(hipEventCreate(&ev_[1]));
+        ENS_TRY(hipMalloc(&d_rec_, rec_total * sizeof(double)));
+        ENS_TRY(hipMalloc(&d_tab_, std::max<size_t>(tab_total, 256)));
+        ENS_TRY(hipMalloc(&d_part_, size_t(3) * parts_total * sizeof(double)));
+        ENS_TRY(hipMalloc(&d_desc_, size_t(n) * sizeof(MemberDesc)));
+        ENS_TRY(hipMalloc(&d_items_, items.size() * sizeof(int2)));
+        ENS_TRY(hipMalloc(&d_counter_, 256));
+        ENS_TRY(hipMalloc(&d_stat_, size_t(6) * n * sizeof(double)));
+        rec_bytes_ = rec_total * sizeof(double);
+        nitems_ = static_cast<int>(items.size());
+        nmembers_ = n;
+        // member descriptors and the work list do not change with the seeds
+        desc_host_.assign(size_t(n) * sizeof(MemberDesc), 0);
+        MemberDesc *desc = reinterpret_cast<MemberDesc *>(desc_host_.data());
+        for (int m = 0; m < n; ++m) {
+            const MfmaPlan &p = plans[m];
+            char *base = static_cast<char *>(d_tab_) + tab_off[m];
+            MemberDesc &d = desc[m];
+            d.xt = base;
+            d.yt = reinterpret_cast<const float2 *>(base + p.x_bytes);
+            d.zt = reinterpret_cast<const float2 *>(base + p.x_bytes + p.y_bytes);
+            d.field = members[m].d_field;
+            d.nx = members[m].g.nx;
+            d.ny = members[m].g.ny;
+            d.nz = members[m].g.nz;
+            d.nx_pad = p.nx_pad;
+            d.stages = p.stages;
+            d.tiles_x = p.tiles_x;
+            d.tiles_j = p.tiles_j;
+            d.tiles_k = p.tiles_k;
+            d.tj_shift = p.tj_shift;
+            d.tk = p.tk;
+            d.nbx = p.nbx;
+            d.part_base = part_base[m];
+            d.scaling = members[m].scaling;
+            d.sigma = members[m].sigma;
+        }
+        ENS_TRY(hipMemcpy(d_desc_, desc_host_.data(), desc_host_.size(), hipMemcpyHostToDevice));
+        ENS_TRY(hipMemcpy(d_items_, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+
+        int dev = 0, cus = 256;
+        ENS_TRY(hipGetDevice(&dev));
+        ENS_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int persistent = std::max(1, std::min(nitems_, cus));
+
+        // capture the launch sequence
+        ENS_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int m = 0; m < n; ++m) {
+            const double *rec = static_cast<const double *>(d_rec_) + rec_off[m];
+            if (!mfma_tables(plans[m], members[m].g, rec, members[m].np_, static_cast<char *>(d_tab_) + tab_off[m],
+                             stream, err)) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                return false;
+            }
+        }
+        reset_counter<<<1, 1, 0, stream>>>(static_cast<int *>(d_counter_));
+        const MemberDesc *dd = static_cast<const MemberDesc *>(d_desc_);
+        const int2 *di = static_cast<const int2 *>(d_items_);
+        int *dc = static_cast<int *>(d_counter_);
+        double *dp = static_cast<double *>(d_part_);
+        switch (s) {
+        case kBf16x3: mfma_ensemble<kBf16x3, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
+        case kBf16x2: mfma_ensemble<kBf16x2, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
+        default: mfma_ensemble<kF16x2s, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
+        }
+        ensemble_finalize<Real><<<n, 256, 0, stream>>>(dd, dp, static_cast<double *>(d_stat_));
+        for (int m = 0; m < n; ++m) {
+            if (members[m].rescale != 1) continue;
+            const long pts = members[m].g.points();
+            const int blocks = int(std::min<long>((pts + 255) / 256, 8192));
+            ensemble_rescale<Real><<<blocks, 256, 0, stream>>>(static_cast<Real *>(members[m].d_field), pts,
+                                                               static_cast<const double *>(d_stat_) + 6 * m);
+        }
+        ENS_TRY(hipStreamEndCapture(stream, &graph_));
+        ENS_TRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        key_ = key;
+        reused_ = false;
+    } else {
+        reused_ = true;
+        // scaling can differ between runs of the same shape (sigma, spectrum)
+        MemberDesc *desc = reinterpret_cast<MemberDesc *>(desc_host_.data());
+        bool changed = false;
+        for (int m = 0; m < n; ++m)
+            if (desc[m].scaling != members[m].scaling) {
+                desc[m].scaling = members[m].scaling;
+                changed = true;
+            }
+        if (changed) ENS_TRY(hipMemcpy(d_desc_, desc_host_.data(), desc_host_.size(), hipMemcpyHostToDevice));
+    }
+    for (int m = 0; m < n; ++m)
+        ENS_TRY(hipMemcpyAsync(static_cast<double *>(d_rec_) + rec_off[m], members[m].record,
+                               size_t(5) * members[m].np_ * sizeof(double), hipMemcpyHostToDevice, stream));
+    ENS_TRY(hipEventRecord(ev_[0], stream));
+    ENS_TRY(hipGraphLaunch(exec_, stream));
+    ENS_TRY(hipEventRecord(ev_[1], stream));
+    stats.assign(size_t(6) * n, 0.0);
+    ENS_TRY(hipMemcpyAsync(stats.data(), d_stat_, size_t(6) * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    ENS_TRY(hipStreamSynchronize(stream));
+    float ms = 0.f;
+    ENS_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    device_ms = ms;
+    return true;
+}
+
+template bool Ensemble::run<float>(hipStream_t, int, const std::vector<EnsembleMember> &, std::vector<double> &,
+                                   double &, std::string &);
+template bool Ensemble::run<double>(hipStream_t, int, const std::vector<EnsembleMember> &, std::vector<double> &,
+                                    double &, std::string &);
 template bool mfma_field<double>(const MfmaPlan &, const Geometry &, const void *, double, double *, double *,
                                  hipStream_t, std::string &);
 

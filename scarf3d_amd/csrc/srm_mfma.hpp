@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "srm_device.hpp"
 
@@ -25,18 +26,67 @@ struct MfmaPlan {
     int stages = 0;         // np_ / harmonics per stage
     int nx_pad = 0;         // x extent padded to the block tile
     int tiles_x = 0;        // block tiles along x
+    int nbx = 4;            // x tile = 32 * nbx points (64, 96 or 128)
     int tj = 0, tk = 0;     // block column tile: tj values of j by tk values of k
+    int tj_shift = 0;       // log2(tj)
     int tiles_j = 0, tiles_k = 0;
     long blocks = 0;
     int nparts = 0;         // statistics partials (one per block)
     size_t x_bytes = 0, y_bytes = 0, z_bytes = 0, table_bytes = 0;
 };
 
-bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err);
+// nbx = 2..4 forces the x tile width (tuner); other values pick it by shape
+bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0);
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err);
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
                 double *d_part, hipStream_t stream, std::string &err);
+
+// One realization of an ensemble, prepared on the host.
+struct EnsembleMember {
+    Geometry g;               // structured grid
+    int np_ = 0;              // padded harmonic count (multiple of 64)
+    const double *record = nullptr;  // host FP64 record, 5 * np_ values (wx, wy, wz, phase, amp)
+    double scaling = 0;
+    int rescale = 0;
+    double sigma = 0;
+    void *d_field = nullptr;  // device output, `Real` elements
+    int nbx = 0;              // x tile override (0: by shape)
+};
+
+// Monte Carlo ensembles: operand tables of every member, one persistent
+// grouped-GEMM launch over all (member, tile) work items pulled from an
+// atomic counter (largest members first), per-member statistics and
+// rescale. The launch sequence is captured once per ensemble shape into a
+// hipGraph and replayed; only the harmonic records change between runs.
+class Ensemble {
+public:
+    Ensemble() = default;
+    ~Ensemble();
+    Ensemble(const Ensemble &) = delete;
+    Ensemble &operator=(const Ensemble &) = delete;
+
+    // stats: 6 values per member {count, mean, variance, info3, info4, rescale factor}
+    template <class Real>
+    bool run(hipStream_t stream, int kernel, const std::vector<EnsembleMember> &members, std::vector<double> &stats,
+             double &device_ms, std::string &err);
+
+    int items() const { return nitems_; }
+    bool graph_reused() const { return reused_; }
+
+private:
+    void release();
+    size_t key_ = 0;
+    bool reused_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    void *d_rec_ = nullptr, *d_tab_ = nullptr, *d_part_ = nullptr, *d_desc_ = nullptr, *d_items_ = nullptr;
+    void *d_counter_ = nullptr, *d_stat_ = nullptr;
+    size_t rec_bytes_ = 0;
+    int nitems_ = 0, nmembers_ = 0;
+    std::vector<unsigned char> desc_host_;
+};
 
 }  // namespace scarf

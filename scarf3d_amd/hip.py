@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -87,6 +88,10 @@ class DeviceArray:
             self.ptr = None
 
     def __del__(self):
+        # at interpreter exit the runtime may already be tearing down: leave
+        # the memory to the process exit instead of calling into HIP then
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:

@@ -34,6 +34,8 @@ GRID_CASES = [
     ("rotated_offset", dict(n=(24, 40, 12), fs=(5, 3, 9), alpha=30.0, beta=-10.0, gamma=5.0, nharm=256)),
     ("dropped_harmonics", dict(n=(20, 20, 20), hurst=0.1, ds=20.0, dh=10.0, nharm=300)),
     ("grid_2d", dict(n=(64, 48, 1), cl=(2000.0, 2000.0, 0.0), nharm=256)),
+    ("x_tile_96", dict(n=(96, 30, 10), nharm=192)),     # 96-point x tiles
+    ("x_tile_128", dict(n=(256, 12, 9), nharm=160)),    # two 128-point x tiles, ragged columns
 ]
 
 
