@@ -100,7 +100,10 @@ int scarf_setup_values(double *out, int n);
  * (src/scarflib.f90:163-175, :276-288): unlike struct scarf_opt, a value of
  * 0 or below counts (e.g. mute = 0). Names: method, hurst, dh, ds, mute,
  * taper, rescale, pad, alpha, beta, gamma, nc1..nc3, fc1..fc3. Used by the
- * Fortran module m_scarflib. */
+ * Fortran module m_scarflib. Two more names set how the host draw runs and
+ * never change the harmonics drawn: draw_threads (workers; 0 = environment
+ * SCARF_DRAW_THREADS or min(16, hardware threads)) and draw_block (stream
+ * pairs per block; 0 = 2^18). */
 int scarf_set_option(const char *name, double value);
 
 /* Parameters held after initialize (defaults applied), for checks:

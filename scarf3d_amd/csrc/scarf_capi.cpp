@@ -42,6 +42,8 @@ struct State {
     const real *x = nullptr, *y = nullptr, *z = nullptr;  // borrowed point coordinates
     int kernel = -1;                                       // -1: not chosen yet (environment / auto)
     int nharm = 10000;
+    int draw_threads = 0;  // host draw workers (0: default)
+    long draw_block = 0;   // pairs per draw block (0: default)
     std::string error;
     std::unique_ptr<scarf::Device> device;
     std::unique_ptr<scarf::Ensemble> ensemble;
@@ -167,8 +169,8 @@ int prepare(int seed)
     const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, err) : scarf::srm_setup<float>(s.cfg, s.setup, err);
     if (rc != 0) return fail(err);
     if (!ensure_device()) return -1;
-    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, s.setup, seed)
-                                          : scarf::srm_draw<float>(s.cfg, s.setup, seed);
+    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, s.setup, seed, s.draw_threads, s.draw_block)
+                                          : scarf::srm_draw<float>(s.cfg, s.setup, seed, s.draw_threads, s.draw_block);
     const scarf::HarmonicTable t = scarf::build_table(s.setup, d);
     s.host_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!s.device->upload_table(t, err)) return fail(err);
@@ -561,8 +563,9 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
                 errs[m] = e;
                 continue;
             }
-            const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed)
-                                                  : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed);
+            // members are drawn in parallel already: one thread per member's draw
+            const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed, 1)
+                                                  : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed, 1);
             const scarf::HarmonicTable t = scarf::build_table(setups[m], d);
             const int np_ = std::max(64, (t.count() + 63) / 64 * 64);
             std::vector<double> &r = records[m];
@@ -676,6 +679,14 @@ int scarf_set_option(const char *name, double value)
     SrmConfig &c = s.cfg;
     const std::string n(name);
     const double v = static_cast<real>(value);
+    // host draw execution (never changes the harmonics drawn)
+    if (n == "draw_threads" || n == "draw_block") {
+        if (value < 0) return fail("scarf_set_option: " + n + " must be >= 0");
+        if (n == "draw_threads") s.draw_threads = static_cast<int>(value);
+        else s.draw_block = static_cast<long>(value);
+        s.error.clear();
+        return 0;
+    }
     if (n == "method") c.method = static_cast<int>(value);
     else if (n == "hurst") c.hurst = v;
     else if (n == "dh") c.dh = v;
@@ -720,8 +731,8 @@ long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
     std::string err;
     const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, su, err) : scarf::srm_setup<float>(s.cfg, su, err);
     if (rc != 0) return fail(err);
-    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, su, seed)
-                                          : scarf::srm_draw<float>(s.cfg, su, seed);
+    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, su, seed, s.draw_threads, s.draw_block)
+                                          : scarf::srm_draw<float>(s.cfg, su, seed, s.draw_threads, s.draw_block);
     for (int l = 0; l < s.cfg.nharm; ++l) {
         harm[5 * l] = d.v1[l];
         harm[5 * l + 1] = d.v2[l];

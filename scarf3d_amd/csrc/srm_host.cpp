@@ -5,6 +5,12 @@
 // shift every later harmonic.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 #include "quadpack.hpp"
 #include "srm_host.hpp"
@@ -44,14 +50,6 @@ Real integrate(const Spectrum<Real> &s, bool finite, Real k0, Real k1)
     const quad::Result r = finite ? quad::dqng(s, static_cast<double>(k0), static_cast<double>(k1), 1.0e-8, 1.0e-5)
                                   : quad::dqagi(s, 0.0, 1, 1.0e-8, 1.0e-5, 500);
     return static_cast<Real>(r.value);
-}
-
-// srng (src/prng.cpp:152-165): two uniforms rounded to `real`
-template <class Real>
-inline void two_uniforms(Yarn2 &g, Real r[2])
-{
-    r[0] = static_cast<Real>(g.uniform01());
-    r[1] = static_cast<Real>(g.uniform01());
 }
 
 }  // namespace
@@ -160,11 +158,138 @@ int srm_setup(const SrmConfig &cfg, SrmSetup &out, std::string &err)
     return 0;
 }
 
+namespace {
+
+// Pairs of uniforms (one srng call each) from index `start` of a seed's
+// stream, with the rejection test of every pair evaluated in advance. Whether
+// a pair is accepted depends on its two values only, never on where the
+// draw loop stands, so the test can run ahead of the loop, in parallel.
+template <class Real>
+struct PairBlock {
+    long start = 0, end = 0;      // pair indices [start, end)
+    std::vector<int64_t> raw;     // 2 generator outputs per pair
+    std::vector<uint8_t> accept;  // rejection test of the pair as a trial
+};
+
+template <class Real>
+struct DrawContext {
+    Spectrum<Real> spec;
+    Real kmin, kmax, kc, cnst;
+    unsigned long seed;
+
+    // the trial of src/scarflib_srm.f90:737-743 on one srng pair
+    Real radial(Real r0) const { return r0 * (kmax - kmin) + kmin; }
+    bool accepts(Real r0, Real r1) const
+    {
+        const Real u = radial(r0);
+        const Real accept = static_cast<Real>(spec(static_cast<double>(u)) * static_cast<double>(cnst));
+        return r1 < accept;
+    }
+    static Real uniform(int64_t raw) { return static_cast<Real>(Yarn2::to_uniform01(raw)); }
+
+    void fill_range(PairBlock<Real> &blk, long first, long last) const
+    {
+        Yarn2 g;
+        g.seed(seed);
+        g.discard(2 * static_cast<uint64_t>(first));
+        for (long p = first; p < last; ++p) {
+            const int64_t a = g(), b = g();
+            const size_t i = static_cast<size_t>(p - blk.start);
+            blk.raw[2 * i] = a;
+            blk.raw[2 * i + 1] = b;
+            blk.accept[i] = accepts(uniform(a), uniform(b)) ? 1 : 0;
+        }
+    }
+};
+
+// Fixed set of worker threads that run one job per call, all threads
+// together (the caller is worker 0).
+class WorkerPool {
+public:
+    explicit WorkerPool(int n) : n_(n)
+    {
+        for (int t = 1; t < n_; ++t) threads_.emplace_back([this, t] { loop(t); });
+    }
+    ~WorkerPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    int size() const { return n_; }
+    void run(const std::function<void(int)> &job)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &job;
+            pending_ = n_ - 1;
+            ++generation_;
+        }
+        cv_.notify_all();
+        job(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void loop(int t)
+    {
+        long seen = 0;
+        for (;;) {
+            const std::function<void(int)> *job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
+                if (stop_) return;
+                seen = generation_;
+                job = job_;
+            }
+            (*job)(t);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    int n_;
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    long generation_ = 0;
+    int pending_ = 0;
+    bool stop_ = false;
+};
+
+}  // namespace
+
+int default_draw_threads()
+{
+    if (const char *e = std::getenv("SCARF_DRAW_THREADS")) {
+        const int v = std::atoi(e);
+        if (v >= 1) return std::min(v, 256);
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(hw, 16u)));
+}
+
 // Harmonic draw (src/scarflib_srm.f90:731-778): rejection sampling of the
 // radial wavenumber against srm_fn*const, a uniform direction, and a, b from
 // the cosine branch of Box-Muller with one srng call each.
+//
+// The loop consumes the seed's stream pair by pair: trials until the first
+// accepted pair q, then pair q+1 (direction) and, for a kept harmonic
+// (u <= kc), pairs q+2 and q+3 (a and b). Here the stream is produced and
+// every pair's trial evaluated ahead of that walk, in blocks, by `threads`
+// workers that each jump to their sub-range (Yarn2::discard); the walk then
+// only looks up the next accepted pair. The pairs consumed, and so every
+// harmonic, are those of the one-pair-at-a-time loop.
 template <class Real>
-HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed)
+HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed, int threads, long block_pairs)
 {
     HarmonicDraw d;
     const int n = cfg.nharm;
@@ -175,30 +300,73 @@ HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed)
     d.b.assign(n, 0.0);
     d.used.assign(n, 0);
 
-    Yarn2 g;
-    g.seed(static_cast<unsigned long>(seed));  // set_seed: unsigned long myseed = seed
-    Spectrum<Real> spec;
-    spec.acf = cfg.acf;
-    spec.ndim = s.ndim;
-    spec.nu = static_cast<Real>(s.nu);
+    DrawContext<Real> ctx;
+    ctx.spec.acf = cfg.acf;
+    ctx.spec.ndim = s.ndim;
+    ctx.spec.nu = static_cast<Real>(s.nu);
+    ctx.kmin = static_cast<Real>(s.kmin);
+    ctx.kmax = static_cast<Real>(s.kmax);
+    ctx.kc = static_cast<Real>(s.kc);
+    ctx.cnst = static_cast<Real>(s.cnst);
+    ctx.seed = static_cast<unsigned long>(seed);  // set_seed: unsigned long myseed = seed
     const Real pi = kPi<Real>;
-    const Real kmin = static_cast<Real>(s.kmin), kmax = static_cast<Real>(s.kmax), kc = static_cast<Real>(s.kc);
-    const Real cnst = static_cast<Real>(s.cnst);
     const Real cl0 = static_cast<Real>(cfg.cl[0]), cl1 = static_cast<Real>(cfg.cl[1]), cl2 = static_cast<Real>(cfg.cl[2]);
-    long draws = 0;
+
+    if (threads <= 0) threads = default_draw_threads();
+    const long max_block = block_pairs > 0 ? block_pairs : (1L << 18);
+    std::unique_ptr<WorkerPool> pool;
+    if (threads > 1) pool.reset(new WorkerPool(threads));
+
+    PairBlock<Real> blk;
+    long next_size = std::min(max_block, std::max(64L, 16L * n));  // first block: a few trials per harmonic
+    // make pairs [first, first + >= 4) resident
+    auto fill = [&](long first) {
+        const long count = std::max(next_size, 4L);
+        next_size = std::min(max_block, 2 * count);
+        blk.start = first;
+        blk.end = first + count;
+        blk.raw.resize(2 * static_cast<size_t>(count));
+        blk.accept.resize(static_cast<size_t>(count));
+        const int nt = (pool && count >= 4096) ? pool->size() : 1;
+        if (nt == 1) {
+            ctx.fill_range(blk, first, first + count);
+            return;
+        }
+        const long chunk = (count + nt - 1) / nt;
+        const std::function<void(int)> job = [&](int t) {
+            const long lo = first + t * chunk, hi = std::min(first + count, lo + chunk);
+            if (lo < hi) ctx.fill_range(blk, lo, hi);
+        };
+        pool->run(job);
+    };
+    auto pair = [&](long p, Real r[2]) {
+        const size_t i = static_cast<size_t>(p - blk.start);
+        r[0] = DrawContext<Real>::uniform(blk.raw[2 * i]);
+        r[1] = DrawContext<Real>::uniform(blk.raw[2 * i + 1]);
+    };
+
+    long pos = 0;  // next pair the loop would read
     Real r[2];
     for (int l = 0; l < n; ++l) {
-        Real u;
-        for (;;) {
-            two_uniforms(g, r);
-            draws += 2;
-            u = r[0] * (kmax - kmin) + kmin;
-            const Real accept = static_cast<Real>(spec(static_cast<double>(u)) * static_cast<double>(cnst));
-            if (r[1] < accept) break;
+        long q = -1;
+        while (q < 0) {
+            if (pos < blk.start || pos >= blk.end) fill(pos);
+            const uint8_t *f = blk.accept.data();
+            for (long p = pos; p < blk.end; ++p)
+                if (f[p - blk.start]) {
+                    q = p;
+                    break;
+                }
+            if (q < 0) pos = blk.end;
         }
-        two_uniforms(g, r);
-        draws += 2;
-        if (u > kc) continue;  // dropped, still counted by the scaling
+        if (q + 3 >= blk.end) fill(q);  // the trial and the (up to) three pairs after it
+        pair(q, r);
+        const Real u = ctx.radial(r[0]);
+        pair(q + 1, r);
+        if (u > ctx.kc) {  // dropped, still counted by the scaling
+            pos = q + 2;
+            continue;
+        }
         const Real phi = r[0] * Real(2) * pi;
         Real v1, v2, v3;
         if (s.ndim == 3) {
@@ -211,11 +379,11 @@ HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed)
             v2 = u * std::sin(phi) / cl1;
             v3 = 0;
         }
-        two_uniforms(g, r);
+        pair(q + 2, r);
         const Real a = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
-        two_uniforms(g, r);
+        pair(q + 3, r);
         const Real b = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
-        draws += 4;
+        pos = q + 4;
         d.v1[l] = v1;
         d.v2[l] = v2;
         d.v3[l] = v3;
@@ -224,7 +392,7 @@ HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed)
         d.used[l] = 1;
         ++d.n_used;
     }
-    d.uniforms = draws;
+    d.uniforms = 2 * pos;
     return d;
 }
 
@@ -254,7 +422,7 @@ HarmonicTable build_table(const SrmSetup &s, const HarmonicDraw &d)
 
 template int srm_setup<float>(const SrmConfig &, SrmSetup &, std::string &);
 template int srm_setup<double>(const SrmConfig &, SrmSetup &, std::string &);
-template HarmonicDraw srm_draw<float>(const SrmConfig &, const SrmSetup &, int);
-template HarmonicDraw srm_draw<double>(const SrmConfig &, const SrmSetup &, int);
+template HarmonicDraw srm_draw<float>(const SrmConfig &, const SrmSetup &, int, int, long);
+template HarmonicDraw srm_draw<double>(const SrmConfig &, const SrmSetup &, int, int, long);
 
 }  // namespace scarf

@@ -67,8 +67,14 @@ struct HarmonicTable {
 template <class Real>
 int srm_setup(const SrmConfig &cfg, SrmSetup &out, std::string &err);
 
+// threads <= 0: default_draw_threads(); block_pairs <= 0: 2^18 pairs per
+// block at most (smaller blocks only change how the work is cut, never the
+// harmonics)
 template <class Real>
-HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &setup, int seed);
+HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &setup, int seed, int threads = 0, long block_pairs = 0);
+
+// SCARF_DRAW_THREADS if set, else min(16, hardware threads)
+int default_draw_threads();
 
 HarmonicTable build_table(const SrmSetup &setup, const HarmonicDraw &draw);
 

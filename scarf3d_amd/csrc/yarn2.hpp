@@ -11,6 +11,7 @@
 // g = 123567893 (which breaks the linear structure of the MRG).
 #pragma once
 #include <cstdint>
+#include <vector>
 
 namespace scarf {
 
@@ -43,17 +44,70 @@ public:
 
     // uniform01_dist<double>: one draw scaled into [0, 1)
     double uniform01() { return static_cast<double>((*this)()) / static_cast<double>(kModulus); }
+    static double to_uniform01(int64_t r) { return static_cast<double>(r) / static_cast<double>(kModulus); }
+
+    // Advance the state by n steps without producing output: the state pair
+    // (r0, r1) evolves by the companion matrix [[a0, a1], [1, 0]] over GF(m),
+    // so n steps are its n-th power (square-and-multiply, O(log n)).
+    void discard(uint64_t n)
+    {
+        uint64_t p[4] = {1, 0, 0, 1};  // accumulated power
+        uint64_t b[4] = {static_cast<uint64_t>(kA0), static_cast<uint64_t>(kA1), 1, 0};
+        while (n > 0) {
+            if (n & 1) mat_mul(p, b);
+            mat_mul(b, b);
+            n >>= 1;
+        }
+        const uint64_t r0 = static_cast<uint64_t>(r0_), r1 = static_cast<uint64_t>(r1_);
+        r0_ = static_cast<int64_t>((p[0] * r0 % kM + p[1] * r1 % kM) % kM);
+        r1_ = static_cast<int64_t>((p[2] * r0 % kM + p[3] * r1 % kM) % kM);
+    }
 
 private:
+    static constexpr uint64_t kM = static_cast<uint64_t>(kModulus);
+
+    static void mat_mul(uint64_t x[4], const uint64_t y[4])  // x = x * y (mod m), y may alias x
+    {
+        const uint64_t r0 = (x[0] * y[0] % kM + x[1] * y[2] % kM) % kM;
+        const uint64_t r1 = (x[0] * y[1] % kM + x[1] * y[3] % kM) % kM;
+        const uint64_t r2 = (x[2] * y[0] % kM + x[3] * y[2] % kM) % kM;
+        const uint64_t r3 = (x[2] * y[1] % kM + x[3] * y[3] % kM) % kM;
+        x[0] = r0;
+        x[1] = r1;
+        x[2] = r2;
+        x[3] = r3;
+    }
+
+    // g^e mod m for 0 < e < m by two table lookups: e = hi * 2^16 + lo,
+    // g^e = g^(hi * 2^16) * g^lo. Exact modular arithmetic, so identical to
+    // square-and-multiply.
+    struct PowTables {
+        std::vector<uint32_t> lo, hi;
+        PowTables() : lo(1u << 16), hi(1u << 15)
+        {
+            uint64_t v = 1;
+            for (uint32_t i = 0; i < (1u << 16); ++i) {
+                lo[i] = static_cast<uint32_t>(v);
+                v = v * static_cast<uint64_t>(kGenerator) % kM;
+            }
+            const uint64_t step = v;  // g^(2^16)
+            v = 1;
+            for (uint32_t i = 0; i < (1u << 15); ++i) {
+                hi[i] = static_cast<uint32_t>(v);
+                v = v * step % kM;
+            }
+        }
+    };
+    static const PowTables &tables()
+    {
+        static const PowTables t;
+        return t;
+    }
     static int64_t pow_generator(int64_t e)
     {
-        uint64_t result = 1, base = static_cast<uint64_t>(kGenerator);
-        while (e > 0) {
-            if (e & 1) result = result * base % static_cast<uint64_t>(kModulus);
-            base = base * base % static_cast<uint64_t>(kModulus);
-            e >>= 1;
-        }
-        return static_cast<int64_t>(result);
+        const PowTables &t = tables();
+        const uint64_t u = static_cast<uint64_t>(e);
+        return static_cast<int64_t>(static_cast<uint64_t>(t.hi[u >> 16]) * t.lo[u & 0xffffu] % kM);
     }
 
     int64_t r0_ = 0;

@@ -76,6 +76,36 @@ def test_setup_and_draw_bit_identical_to_oracle(oracle, lib_f, lib_d, flavour, c
         assert np.array_equal(h1, h2)
 
 
+@pytest.mark.parametrize("flavour", ["f", "d"])
+@pytest.mark.parametrize("threads,block", [(1, 0), (1, 5), (3, 4099), (8, 4096), (16, 0)])
+def test_parallel_draw_consumes_the_same_stream(oracle, lib_f, lib_d, flavour, threads, block):
+    """The host draw evaluates the stream's trials ahead of the loop, in
+    blocks, on several threads that jump into the stream; whatever the cut
+    (a block of 5 pairs splits most harmonics across blocks; 4099 pairs over
+    3 threads leaves ragged chunks), the harmonics and the number of uniforms
+    consumed are those of the oracle's one-pair-at-a-time loop."""
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
+    cfg = structured_cfg(n=(64, 64, 64), ds=20.0, hurst=0.1, cl=(5000.0,) * 3, nharm=1500, real=real)
+    p = oracle.params(cfg)
+    s = oracle.setup(p, flavour)
+    init_product(lib, cfg)
+    assert lib.set_option("draw_threads", threads) == 0
+    assert lib.set_option("draw_block", block) == 0
+    for seed in (1234, 98765):
+        h1, u1, n1 = oracle.draw(p, s, seed, flavour)
+        h2, u2, n2 = lib.draw_harmonics(seed, 1500)
+        assert n1 == n2 and n1 > 100 * 1500  # low acceptance: a long stream
+        assert np.array_equal(u1, u2)
+        assert np.array_equal(h1, h2)
+
+
+def test_draw_options_rejected_when_negative(lib_f):
+    init_product(lib_f, structured_cfg(nharm=64))
+    assert lib_f.set_option("draw_threads", -1) != 0
+    assert "draw_threads" in lib_f.last_error()
+
+
 def test_unstructured_setup_bit_identical(oracle, lib_f):
     cfg = unstructured_cfg(npts=500, nharm=200)
     p = oracle.params(cfg)
