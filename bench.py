@@ -34,16 +34,35 @@ sys.path.insert(0, str(ROOT / "tests"))
 METRIC = "SRM Gpt·harm/s at 512³×20k: split-BF16 MFMA vs FP32 direct, err≤1e-5σ"
 PEAK_MFMA_TFLOPS = 2500.0  # MI355X dense bf16 / fp16
 PEAK_VALU_TFLOPS = 157.3   # MI355X FP32 vector
+PEAK_VALU_F64_TFLOPS = 78.6  # MI355X FP64 vector
 MFMA_PRODUCTS = {"mfma_bf16x3": 6, "mfma_f16x2": 3, "mfma_bf16x2": 3}
 
+# Workloads. flavour: library `real` (f = libscarf3df, d = libscarf3d);
+# kernel: the timed field kernel; compare: kernels also timed on the same
+# resident table; every field is checked against the FP64 kernel over the
+# whole field and against the FP64 CPU oracle at 256 sampled points.
 WORKLOADS = {
+    # 512^3 x 20k, the configuration the metric is quoted on (seed 1234)
     "precision_512": dict(n=(512, 512, 512), ds=20.0, acf=0, hurst=0.1, cl=(5000.0,) * 3, sigma=0.05, nharm=20000,
-                          seed=1234),
+                          seed=1234, flavour="f", kernel="mfma_bf16x3",
+                          compare=("direct_f32", "mfma_f16x2", "mfma_bf16x2")),
+    # 128^3 x 5k with FP64 host semantics (double library); the reference's
+    # CPU path on one rank is timed beside the GPU FP32 direct-sum kernel
     "plain_128": dict(n=(128, 128, 128), ds=50.0, acf=0, hurst=0.25, cl=(2000.0,) * 3, sigma=0.05, nharm=5000,
-                      seed=1234),
+                      seed=1234, flavour="d", kernel="direct_f32", compare=("mfma_bf16x3", "mfma_f16x2", "direct_f64"),
+                      cpu_one_rank=True),
+    # 2,000,000 points uniform in 10 x 10 x 5 km (numpy PCG64 seed 7, rounded
+    # to float), Gaussian cl = (3000, 3000, 1000) m, dh = 50 m, 10k harmonics
+    "points_2m": dict(points=2_000_000, box=(10000.0, 10000.0, 5000.0), ds=50.0, acf=1, hurst=0.25,
+                      cl=(3000.0, 3000.0, 1000.0), sigma=0.1, nharm=10000, seed=7, flavour="f", kernel="direct_f32",
+                      compare=("direct_f64",)),
+    # 1024 x 1024 x 512 x 50k, 16 POIs (numpy seed 16), mute 20 m, taper 500 m,
+    # rescale to sigma (seed 1234)
     "large_1024": dict(n=(1024, 1024, 512), ds=20.0, acf=0, hurst=0.2, cl=(6000.0, 4000.0, 1500.0), sigma=0.08,
-                       nharm=50000, seed=1234, rescale=1, npoi=16, mute=20.0, taper=500.0),
+                       nharm=50000, seed=1234, rescale=1, npoi=16, mute=20.0, taper=500.0, flavour="f",
+                       kernel="mfma_bf16x3", compare=("mfma_f16x2",)),
 }
+KERNELS = ("mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2", "direct_f32", "direct_f64")
 
 
 def _sig(v, digits=4):
@@ -143,15 +162,20 @@ def log(*a):
 
 
 def init(lib, w):
-    from srm_cases import init_product, structured_cfg
+    from srm_cases import init_product, structured_cfg, unstructured_cfg
 
-    kw = {k: w[k] for k in ("n", "ds", "acf", "hurst", "cl", "sigma", "nharm") if k in w}
-    extra = {}
-    if w.get("npoi"):
-        rng = np.random.default_rng(16)
-        ext = np.array([(w["n"][i] - 1) * w["ds"] for i in range(3)])
-        extra = dict(poi=(rng.uniform(0, 1, (3, w["npoi"])) * ext[:, None]), mute=w["mute"], taper=w["taper"])
-    cfg = structured_cfg(rescale=w.get("rescale", 0), **kw, **extra)
+    real = np.float32 if w["flavour"] == "f" else np.float64
+    if "points" in w:
+        cfg = unstructured_cfg(npts=w["points"], box=w["box"], dh=w["ds"], acf=w["acf"], cl=w["cl"], sigma=w["sigma"],
+                               hurst=w["hurst"], nharm=w["nharm"], seed=w["seed"], real=real)
+    else:
+        kw = {k: w[k] for k in ("n", "ds", "acf", "hurst", "cl", "sigma", "nharm") if k in w}
+        extra = {}
+        if w.get("npoi"):
+            rng = np.random.default_rng(16)
+            ext = np.array([(w["n"][i] - 1) * w["ds"] for i in range(3)])
+            extra = dict(poi=(rng.uniform(0, 1, (3, w["npoi"])) * ext[:, None]), mute=w["mute"], taper=w["taper"])
+        cfg = structured_cfg(rescale=w.get("rescale", 0), real=real, **kw, **extra)
     init_product(lib, cfg)
     return cfg
 
@@ -174,43 +198,105 @@ def time_kernel(lib, hip, kernel, d_field, steps, warmup):
                 stats=stats)
 
 
-def cpu_baseline(oracle, cfg, seed, seconds_target=15.0):
-    """The reference loop (harmonic outer, points inner, float sin/cos) on
-    the host cores, over a bounded z-slab subset of the same workload."""
+def _host_threads():
+    return min(16, len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(oracle, cfg, seed, flavour="f", threads=None, seconds_target=15.0):
+    """The reference loop (harmonic outer, points inner, libm sin/cos at
+    `real` precision) on the host cores, each thread owning a block of
+    z-slabs (or of points) like an MPI rank, over a bounded subset of the
+    same workload: calibrated on a short run, then sized to about
+    seconds_target."""
     import ctypes as C
 
     from oracle_lib import _DP
 
     p = oracle.params(cfg)
-    s = oracle.setup(p, "f")
-    harm, used, _ = oracle.draw(p, s, seed, "f")
-    threads = min(16, len(os.sched_getaffinity(0)))
-    nx, ny = cfg["fe"][0], cfg["fe"][1]
+    s = oracle.setup(p, flavour)
+    harm, used, _ = oracle.draw(p, s, seed, flavour)
+    threads = threads or _host_threads()
     nh = int(used.sum())
+    structured = cfg.get("structured", True)
+    if structured:
+        nx, ny, nz = cfg["fe"][0], cfg["fe"][1], cfg["fe"][2]
+        unit, total = nx * ny, nz  # points per slab, slabs
+    else:
+        unit, total = 1, p.npts
+    real = np.float32 if flavour == "f" else np.float64
 
-    def run(nslab, nharm_used):
+    def run(nunits, nharm_used):
         u = used.copy()
         idx = np.flatnonzero(u)
         u[idx[nharm_used:]] = 0
-        out = np.zeros(nx * ny * nslab, dtype=np.float32)
+        out = np.zeros(unit * nunits, dtype=real)
         t0 = time.perf_counter()
-        oracle.L.orc_baseline_structured_f(C.byref(p), C.byref(s), harm.ctypes.data_as(_DP),
-                                           u.ctypes.data_as(C.POINTER(C.c_uint8)), 0, nslab,
-                                           out.ctypes.data_as(C.c_void_p), threads)
+        if structured:
+            fn = getattr(oracle.L, f"orc_baseline_structured_{flavour}")
+            fn(C.byref(p), C.byref(s), harm.ctypes.data_as(_DP), u.ctypes.data_as(C.POINTER(C.c_uint8)), 0, nunits,
+               out.ctypes.data_as(C.c_void_p), threads)
+        else:
+            oracle.L.orc_baseline_unstructured_f(C.byref(p), C.byref(s), harm.ctypes.data_as(_DP),
+                                                 u.ctypes.data_as(C.POINTER(C.c_uint8)), C.c_long(0),
+                                                 C.c_long(nunits), out.ctypes.data_as(C.c_void_p), threads)
         return time.perf_counter() - t0
 
-    # calibrate on a small slice, then run a sample of about seconds_target
+    probe_units = min(total, threads if structured else 200_000)
     probe_h = min(nh, 200)
-    tp = run(threads, probe_h)
-    rate = nx * ny * threads * probe_h / tp
-    nslab = max(threads, int(round(seconds_target * rate / (nx * ny * nh) / threads)) * threads)
-    nslab = min(nslab, cfg["fe"][2])
-    hh = nh if nslab >= threads else probe_h
-    t = run(nslab, hh)
-    wps = nx * ny * nslab * hh / t
+    tp = run(probe_units, probe_h)
+    rate = unit * probe_units * probe_h / tp
+    n_units = int(seconds_target * rate / (unit * nh))
+    step = threads if structured else 1
+    n_units = min(total, max(step, n_units // step * step))
+    hh = nh
+    if unit * n_units * nh / rate > 4 * seconds_target:  # even the smallest subset is too long: cut harmonics
+        hh = max(probe_h, int(seconds_target * rate / (unit * n_units)))
+    t = run(n_units, hh)
+    wps = unit * n_units * hh / t
+    what = f"{n_units} z-slabs of {nx}x{ny}" if structured else f"{n_units} points"
     return dict(value=wps / 1e9, unit="Gpt·harm/s", cores=threads, kind="port",
-                sample=f"{nslab} z-slabs of {nx}x{ny} x {hh} harmonics ({t:.1f} s), reference loop order, FP32 libm "
-                       f"sin/cos, {threads} OpenMP threads each owning z-slabs")
+                sample=f"{what} x {hh} harmonics ({t:.1f} s), reference loop order, "
+                       f"{'FP32' if flavour == 'f' else 'FP64'} libm sin/cos, {threads} OpenMP thread(s) each owning "
+                       f"{'z-slabs' if structured else 'a point range'}")
+
+
+def oracle_points(oracle, cfg, seed, flavour, nsamp=256):
+    """FP64 CPU oracle at sampled points of the workload (same draw as the
+    library): returns (flat indices, values)."""
+    p = oracle.params(cfg)
+    s = oracle.setup(p, flavour)
+    harm, used, _ = oracle.draw(p, s, seed, flavour)
+    rng = np.random.default_rng(5)
+    if cfg.get("structured", True):
+        n = [cfg["fe"][a] - cfg["fs"][a] + 1 for a in range(3)]
+        ii, jj, kk = (rng.integers(0, n[a], nsamp) for a in range(3))
+        ref = oracle.field_at(p, s, harm, used, ii, jj, kk)
+        return ii + n[0] * (jj + n[1] * kk), ref
+    flat = rng.integers(0, p.npts, nsamp)
+    sub = dict(cfg, x=cfg["x"][flat], y=cfg["y"][flat], z=cfg["z"][flat])
+    p2 = oracle.params(sub)  # same parameters, the sampled points only
+    return flat, oracle.field_unstructured(p2, s, harm, used)
+
+
+def epilogue_at(vals, flat, cfg, mean, var):
+    """Rescale (src/scarflib_srm.f90:897-912: field * sigma/sqrt(var) - mean)
+    and the taper/mute of every POI in order (scarflib_common.f90:881-941) at
+    the structured points `flat`, in FP64, given the FP64 field's moments."""
+    out = np.array(vals, dtype=np.float64)
+    if cfg.get("rescale") == 1:
+        out = out * (cfg["sigma"] / np.sqrt(var)) - mean
+    poi = cfg.get("poi")
+    if poi is not None and cfg.get("structured", True):
+        n = [cfg["fe"][a] - cfg["fs"][a] + 1 for a in range(3)]
+        idx = [flat % n[0], (flat // n[0]) % n[1], flat // (n[0] * n[1])]
+        pts = np.stack([(idx[a] + cfg["fs"][a] - 1) * cfg["ds"] for a in range(3)])
+        mute, taper = cfg["mute"], cfg["taper"]
+        for q in range(np.asarray(poi).shape[1]):
+            d = np.sqrt(((pts - np.asarray(poi)[:, q:q + 1]) ** 2).sum(axis=0))
+            f = np.where(d <= mute, 0.0, np.where(d <= mute + taper,
+                                                  0.5 - 0.5 * np.cos(np.pi * (d - mute) / taper), 1.0))
+            out = out * f
+    return out
 
 
 def main(argv=None) -> int:
@@ -219,9 +305,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS) + ["ensemble_64", "copy_calib"])
-    ap.add_argument("--kernel", default="mfma_bf16x3", choices=["mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2",
-                                                                 "direct_f32"])
-    ap.add_argument("--no-compare", action="store_true", help="skip the direct-sum and FP64 reference runs")
+    ap.add_argument("--kernel", default=None, choices=KERNELS, help="timed kernel (default: the workload's)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the comparison kernels and the FP64 reference")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -230,6 +315,7 @@ def main(argv=None) -> int:
         return 2
 
     if args.workload == "ensemble_64":
+        args.kernel = args.kernel or "mfma_bf16x3"
         return bench_ensemble(args)
     if args.workload == "copy_calib":
         return bench_copy_calibration(args)
@@ -239,108 +325,132 @@ def main(argv=None) -> int:
     from scarf3d_amd.hip import Hip
 
     w = WORKLOADS[args.workload]
-    lib = scarf3d_amd.load("f")
+    kernel = args.kernel or w["kernel"]
+    flavour = w["flavour"]
+    real = np.float32 if flavour == "f" else np.float64
+    lib = scarf3d_amd.load(flavour)
     cfg = init(lib, w)
-    npts = int(np.prod(w["n"]))
+    structured = cfg.get("structured", True)
+    npts = int(np.prod(w["n"])) if structured else int(w["points"])
     work = float(npts) * w["nharm"]
     hip = Hip()
-    d_field = hip.empty(npts, np.float32)
+    d_field = hip.empty(npts, real)
     n_used, host_s = lib.srm_draw(w["seed"])
-    log(f"[bench] {args.workload}: {w['n']} x {w['nharm']} harmonics ({n_used} kept), host draw {host_s:.2f} s")
+    shape = w["n"] if structured else f"{npts} points"
+    log(f"[bench] {args.workload}: {shape} x {w['nharm']} harmonics ({n_used} kept), host draw {host_s:.3f} s")
 
-    res = {args.kernel: time_kernel(lib, hip, args.kernel, d_field, args.steps, args.warmup)}
+    res = {kernel: time_kernel(lib, hip, kernel, d_field, args.steps, args.warmup)}
     main_field = d_field.numpy()
-    log(f"[bench] {args.kernel}: {res[args.kernel]['field_ms']:.2f} ms field kernel, "
-        f"{res[args.kernel]['ms_per_step']:.2f} ms/step")
+    log(f"[bench] {kernel}: {res[kernel]['field_ms']:.3f} ms field kernel, {res[kernel]['ms_per_step']:.3f} ms/step")
 
     extra = {}
     if not args.no_compare:
-        others = [k for k in ("direct_f32", "mfma_f16x2", "mfma_bf16x2") if k != args.kernel]
-        for k in others:
-            steps = max(1, min(3, args.steps)) if k == "direct_f32" else args.steps
+        for k in w["compare"]:
+            if k == kernel:
+                continue
+            slow = k in ("direct_f32", "direct_f64") and work > 1e12
+            steps = max(1, min(3, args.steps)) if slow else args.steps
             res[k] = time_kernel(lib, hip, k, d_field, steps, 1)
-            log(f"[bench] {k}: {res[k]['field_ms']:.2f} ms field kernel")
+            log(f"[bench] {k}: {res[k]['field_ms']:.3f} ms field kernel")
             res[k]["field"] = d_field.numpy()
+        d_field.free()
         # FP64 reference over the whole field, on the same resident table
         t0 = time.perf_counter()
         d_ref = hip.empty(npts, np.float64)
-        lib.srm_run_fp64(d_ref.ptr)
+        st64 = lib.srm_run_fp64(d_ref.ptr)  # {count, mean, variance, sqrt} of the field before the epilogue
         hip.synchronize()
         ref64 = d_ref.numpy()
         d_ref.free()
         log(f"[bench] FP64 reference field in {time.perf_counter() - t0:.1f} s")
         sigma = cfg["sigma"]
-        res[args.kernel]["field"] = main_field
+        res[kernel]["field"] = main_field
         errs = {}
         for k, r in res.items():
             dif = r["field"].astype(np.float64) - ref64
-            errs[k] = dict(max_err_sigma=float(np.abs(dif).max() / sigma),
-                           rms_err_sigma=float(np.sqrt(np.mean(dif * dif)) / sigma))
+            st3 = float(r["stats"][2])
+            mx, rms = float(np.abs(dif).max()), float(np.sqrt(np.mean(dif * dif)))
+            errs[k] = dict(max_err_sigma=mx / sigma, rms_err_sigma=rms / sigma, max_err_stats3=mx / st3)
             del dif
             log(f"[bench] {k}: max|err| = {errs[k]['max_err_sigma']:.2e} sigma, rms {errs[k]['rms_err_sigma']:.2e}")
-        # the FP64 kernel against the CPU oracle at sampled points (same float-flavour draw)
-        oracle = Oracle()
-        p = oracle.params(cfg)
-        s = oracle.setup(p, "f")
-        harm, used, _ = oracle.draw(p, s, w["seed"], "f")
-        rng = np.random.default_rng(5)
-        nsamp = 256
-        ii, jj, kk = (rng.integers(0, w["n"][a], nsamp) for a in range(3))
-        ref_pts = oracle.field_at(p, s, harm, used, ii, jj, kk)
-        flat = ii + w["n"][0] * (jj + w["n"][1] * kk)
-        main_pts = main_field[flat].astype(np.float64)
+        # the FP64 kernel and the timed kernel against the CPU oracle at sampled points (same draw)
+        flat, ref_pts = oracle_points(Oracle(), cfg, w["seed"], flavour)
+        ref_pts = epilogue_at(ref_pts, flat, cfg, st64[1], st64[2])
         extra["oracle_points"] = dict(
-            n=nsamp, note="FP64 CPU oracle vs the float-library field at sampled points (same draw)",
-            max_err_sigma=float(np.abs(main_pts - ref_pts).max() / sigma))
-        f64_pts = ref64[flat]
-        extra["fp64_kernel_vs_oracle_max_err_sigma"] = float(np.abs(f64_pts - ref_pts).max() / sigma)
+            n=len(flat), note="FP64 CPU oracle vs the timed kernel's field at sampled points (same draw)",
+            max_err_sigma=float(np.abs(main_field[flat].astype(np.float64) - ref_pts).max() / sigma))
+        extra["fp64_kernel_vs_oracle_max_err_sigma"] = float(np.abs(ref64[flat] - ref_pts).max() / sigma)
         log(f"[bench] oracle points: {extra['oracle_points']['max_err_sigma']:.2e} sigma; FP64 kernel vs oracle "
             f"{extra['fp64_kernel_vs_oracle_max_err_sigma']:.2e} sigma")
         del ref64
         for k in res:
             res[k].pop("field", None)
             res[k].update(errs.get(k, {}))
+    else:
+        d_field.free()
 
-    main_r = res[args.kernel]
-    P = MFMA_PRODUCTS.get(args.kernel)
+    main_r = res[kernel]
+    P = MFMA_PRODUCTS.get(kernel)
     if P:
-        flops = work * 4 * P
-        achieved = flops / (main_r["field_ms"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_MFMA_TFLOPS, 4), "traffic": None,
-                "kernel": f"mfma_grid<{args.kernel}>", "flops_per_point_harmonic": 4 * P,
-                "useful_frac": round(work * 4 / (main_r["field_ms"] * 1e-3) / 1e12 / PEAK_MFMA_TFLOPS, 4)}
+        achieved = work * 4 * P / (main_r["field_ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": _sig(achieved), "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": _sig(achieved / PEAK_MFMA_TFLOPS), "traffic": None,
+                "kernel": f"mfma_grid<{kernel}>", "flops_per_point_harmonic": 4 * P,
+                "useful_frac": _sig(work * 4 / (main_r["field_ms"] * 1e-3) / 1e12 / PEAK_MFMA_TFLOPS)}
+    elif kernel == "direct_f64":
+        achieved = work * 10 / (main_r["field_ms"] * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": _sig(achieved), "peak": PEAK_VALU_F64_TFLOPS, "unit": "TFLOP/s",
+                "frac": _sig(achieved / PEAK_VALU_F64_TFLOPS), "traffic": None,
+                "kernel": "direct_grid_f64" if structured else "direct_points<f64>", "flops_per_point_harmonic": 10}
     else:
         achieved = work * 10 / (main_r["field_ms"] * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_VALU_TFLOPS, 4), "traffic": None, "kernel": "direct_grid_f32",
-                "flops_per_point_harmonic": 10}
+        roof = {"bound": "valu", "achieved": _sig(achieved), "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": _sig(achieved / PEAK_VALU_TFLOPS), "traffic": None,
+                "kernel": "direct_grid_f32" if structured else "direct_points<f32>", "flops_per_point_harmonic": 10}
 
     cpu = None
     if not args.no_cpu:
         oracle = Oracle()
-        cpu = cpu_baseline(oracle, cfg, w["seed"])
-        log(f"[bench] CPU baseline {cpu['value']:.3f} Gpt.harm/s on {cpu['cores']} threads ({cpu['sample']})")
+        cpu = cpu_baseline(oracle, cfg, w["seed"], flavour)
+        log(f"[bench] CPU baseline {cpu['value']:.4f} Gpt.harm/s on {cpu['cores']} threads ({cpu['sample']})")
+        if w.get("cpu_one_rank"):
+            one = cpu_baseline(oracle, cfg, w["seed"], flavour, threads=1)
+            extra["cpu_baseline_one_rank"] = one
+            log(f"[bench] CPU baseline, one rank: {one['value']:.4f} Gpt.harm/s ({one['sample']})")
 
     value = work / (main_r["ms_per_step"] * 1e-3) / 1e9
+    dtype = {"mfma_bf16x3": "bf16x3-split MFMA, fp32 accumulate", "mfma_f16x2": "fp16x2-split (scaled) MFMA, fp32 "
+             "accumulate", "mfma_bf16x2": "bf16x2-split MFMA, fp32 accumulate", "direct_f32": "f32 (FP64 phase "
+             "reduction)", "direct_f64": "f64"}[kernel]
+    config = {"workload": args.workload, "harmonics": w["nharm"], "dh_m": w["ds"],
+              "acf": "von Karman" if w["acf"] == 0 else "Gaussian", "cl_m": list(w["cl"]), "sigma": w["sigma"],
+              "seed": w["seed"], "kernel": kernel, "library": "libscarf3df" if flavour == "f" else "libscarf3d",
+              "host_draw_s": _sig(host_s)}
+    if structured:
+        config["grid"] = list(w["n"])
+    else:
+        config["points"] = npts
+        config["box_m"] = list(w["box"])
+    if w["acf"] == 0:
+        config["hurst"] = w["hurst"]
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Gpt·harm/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(main_r["ms_per_step"], 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3-split MFMA, fp32 accumulate" if P == 6 else args.kernel,
+        "warmup": args.warmup, "ms_per_step": _sig(main_r["ms_per_step"]), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (harmonics drawn on the host from the configured spectrum)",
-        "config": {"workload": args.workload, "grid": list(w["n"]), "harmonics": w["nharm"], "dh_m": w["ds"],
-                   "acf": "von Karman" if w["acf"] == 0 else "Gaussian", "hurst": w["hurst"], "cl_m": list(w["cl"]),
-                   "sigma": w["sigma"], "seed": w["seed"], "kernel": args.kernel, "host_draw_s": round(host_s, 3)},
+        "config": config,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": {k: {kk: (_sig(vv) if isinstance(vv, float) else vv) for kk, vv in r.items() if kk != "stats"}
                     for k, r in res.items()},
     }
-    if "direct_f32" in res and args.kernel != "direct_f32":
-        out["speedup_vs_fp32_direct"] = round(res["direct_f32"]["field_ms"] / main_r["field_ms"], 2)
-        out["kernels"]["direct_f32"]["gpt_harm_per_s"] = round(work / (res["direct_f32"]["field_ms"] * 1e-3) / 1e9, 2)
+    if args.workload != "precision_512":
+        out["metric"] = f"SRM Gpt·harm/s, workload {args.workload}"
+    if "direct_f32" in res and kernel != "direct_f32":
+        out["speedup_vs_fp32_direct"] = _sig(res["direct_f32"]["field_ms"] / main_r["field_ms"])
     for k, r in out["kernels"].items():
-        r.setdefault("gpt_harm_per_s", round(work / (r["field_ms"] * 1e-3) / 1e9, 2))
+        r.setdefault("gpt_harm_per_s", _sig(work / (r["field_ms"] * 1e-3) / 1e9))
+        if "direct_f32" in res and k != "direct_f32":
+            r["speedup_vs_fp32_direct"] = _sig(res["direct_f32"]["field_ms"] / r["field_ms"])
     out.update(extra)
     print(json.dumps(out), flush=True)
     lib.finalize()

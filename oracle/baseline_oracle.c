@@ -97,3 +97,33 @@ void orc_baseline_structured_d(const orc_params *p, const orc_setup *s, const do
     }
 }
 
+/* CPU baseline on a point list: the reference's unstructured loop nest at
+ * float precision (:399-405, :431-436), each thread owning a contiguous range
+ * of points [i0, i1) like an MPI rank. */
+void orc_baseline_unstructured_f(const orc_params *p, const orc_setup *s, const double *harm,
+                                 const uint8_t *used, long i0, long i1, float *field, int threads)
+{
+    float m[9], bar[3];
+    for (int i = 0; i < 9; ++i) m[i] = (float)s->matrix[i];
+    for (int i = 0; i < 3; ++i) bar[i] = (float)s->bar[i];
+    const float scaling = (float)s->scaling;
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+    for (int t = 0; t < threads; ++t) {
+        const long span = i1 - i0;
+        const long pb = i0 + span * t / threads, pe = i0 + span * (t + 1) / threads;
+        for (int l = 0; l < p->nharm; ++l) {
+            if (!used[l]) continue;
+            const double *h = harm + 5 * l;
+            const float v1 = (float)h[0], v2 = (float)h[1], v3 = (float)h[2], a = (float)h[3], b = (float)h[4];
+            for (long q = pb; q < pe; ++q) {
+                const float x = (float)p->x[q], y = (float)p->y[q], z = (float)p->z[q];
+                const float arg = v1 * ((m[0] * x + m[3] * y + m[6] * z) + bar[0]) +
+                                 v2 * ((m[1] * x + m[4] * y + m[7] * z) + bar[1]) +
+                                 v3 * ((m[2] * x + m[5] * y + m[8] * z) + bar[2]);
+                field[q - i0] = field[q - i0] + a * sinf(arg) + b * cosf(arg);
+            }
+        }
+        for (long q = pb; q < pe; ++q) field[q - i0] = field[q - i0] * scaling;
+    }
+}

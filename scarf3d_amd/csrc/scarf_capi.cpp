@@ -594,6 +594,7 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
         em[m].g = geometry(cfgs[m]);
         em[m].np_ = nps[m];
         em[m].record = records[m].data();
+        for (int l = 0; l < nps[m]; ++l) em[m].amp_max = std::max(em[m].amp_max, records[m][4 * size_t(nps[m]) + l]);
         em[m].scaling = setups[m].scaling;
         em[m].rescale = cfgs[m].rescale;
         em[m].sigma = cfgs[m].sigma;

@@ -85,6 +85,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[6] = {};
     int nharm_ = 0;
+    double amp_max_ = 0;  // largest amplitude of the resident table
     // harmonic record: wx, wy, wz (rad/m), phase (rad), amp; plus revolution copies
     double *d_harm_ = nullptr;
     size_t harm_cap_ = 0;

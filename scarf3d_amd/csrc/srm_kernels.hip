@@ -522,6 +522,8 @@ bool Device::upload_table(const HarmonicTable &t, std::string &err)
     HIP_TRY(hipMemcpyAsync(d_harm_, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     nharm_ = np_;
+    amp_max_ = 0;
+    for (int l = 0; l < n; ++l) amp_max_ = std::max(amp_max_, t.amp[l]);
     return true;
 }
 
@@ -567,6 +569,7 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
         if (mfma) {
             MfmaPlan plan;
             if (!mfma_plan(g, np_, kernel, plan, err)) return false;
+            plan.zscale = mfma_zscale(kernel, amp_max_);
             void *p = d_tab_;
             if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;
             d_tab_ = p;

@@ -1,18 +1,16 @@
 // MFMA formulation of the structured-grid SRM (see srm_mfma.hpp).
 //
-// Block: 512 threads = 8 waves, output tile 256 grid columns x 128 x-points.
-// Wave w owns 32 columns and all 128 x: four v_mfma_f32_32x32x16 blocks.
-// Per stage of 16 harmonics (K = 32):
-//   - the x-operand tile (128 x 32 K per part, pre-split at table time) is
-//     copied global -> LDS once per block and read by all 8 waves;
-//   - the generated operand G[c, K] = Re/Im(Z[n,k] Y[n,j]) is formed by the
-//     loader threads as FP32 complex products in LDS (one per column and
-//     harmonic), and each lane then splits its own 8 K values into 16-bit
-//     parts directly in MFMA operand layout (row c = lane % 32,
+// Block: 512 threads = 8 waves, output tile 256 grid columns x TX x-points
+// (TX = 32 * NBX = 64, 96 or 128). Wave w owns 32 columns and all TX x:
+// NBX v_mfma_f32_32x32x16 blocks. Per stage of 16 harmonics (K = 32):
+//   - the x-operand tile (TX x 32 K per part, pre-split at table time) and
+//     the raw complex factors Y (16 x tj) and Z (16 x tk) move global -> LDS
+//     by LDS DMA (global_load_lds), issued one stage ahead;
+//   - each lane forms the generated operand G[c, K] = Re/Im(Z[n,k] Y[n,j])
+//     of its own column and 8 K values and splits it into 16-bit parts
+//     directly in MFMA operand layout (row c = lane % 32,
 //     K = 8 (lane / 32) .. +7);
-//   - next-stage global loads are issued before the MFMAs of the current
-//     stage and written to the other LDS buffer after them (one barrier per
-//     stage).
+//   - one vmcnt wait and one barrier end each stage.
 // Accumulation: FP32 MFMA accumulators, promoted into a second FP32
 // accumulator every 128 harmonics (8 stages), so the rounding of the long
 // sum stays ~1e-7 of the field's standard deviation.
@@ -20,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "srm_mfma.hpp"
@@ -61,12 +60,16 @@ struct SchemeOf<kBf16x2> {
 template <>
 struct SchemeOf<kF16x2s> {
     static constexpr int parts = 2;
-    static constexpr int accs = 2;
+    static constexpr int accs = 1;
     using vec = f16x8;
     using elt = _Float16;
 };
 
-constexpr float kLoScale = 4096.f;  // fp16 low parts are stored times 2^12
+// fp16 low parts are stored times 2^12 (out of the subnormal range). The
+// high part of the generated operand is multiplied by 2^12 too before its
+// product with the x high part, so all three products carry the same 2^12 and
+// share one accumulator; |G| < 8 (mfma_zscale) keeps 2^12 G inside fp16.
+constexpr float kLoScale = 4096.f;
 
 // split x into `parts` 16-bit values whose sum reproduces x to 2^-24 (bf16
 // x3), 2^-16 (bf16 x2) or 2^-22 (fp16 x2, low part scaled by 2^12)
@@ -105,7 +108,8 @@ __device__ inline int chunk_slot(int r, int q) { return r * 4 + (q ^ ((r >> 1) &
 template <int S>
 __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restrict__ harm, int np_, int nx,
                                                            int nx_pad, int ny, int nz, int ox, int oy, int oz,
-                                                           double ds, typename SchemeOf<S>::elt *__restrict__ xt,
+                                                           double ds, double zscale,
+                                                           typename SchemeOf<S>::elt *__restrict__ xt,
                                                            float2 *__restrict__ yt, float2 *__restrict__ zt)
 {
     using E = typename SchemeOf<S>::elt;
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restr
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
-            const double a = harm[4 * np_ + n];
+            const double a = harm[4 * np_ + n] * zscale;
             zt[size_t(n) * nz + k] = make_float2(static_cast<float>(a * cd), static_cast<float>(a * sd));
         }
     }
@@ -285,6 +289,8 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
             for (int p = 0; p < P; ++p)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) g[p][e] = gp[p][e];
+            V g0s{};
+            if constexpr (S == kF16x2s) g0s = g[0] * static_cast<E>(kLoScale);  // exact: power of two, |G| < 8
 #pragma unroll
             for (int bx = 0; bx < NBX; ++bx) {
                 const int row = bx * 32 + (lane & 31);
@@ -304,17 +310,17 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
                     acc[0][bx] = mma(g[1], xv[0], acc[0][bx]);
                     acc[0][bx] = mma(g[0], xv[0], acc[0][bx]);
                 } else {
-                    acc[1][bx] = mma(g[0], xv[1], acc[1][bx]);
-                    acc[1][bx] = mma(g[1], xv[0], acc[1][bx]);
-                    acc[0][bx] = mma(g[0], xv[0], acc[0][bx]);
+                    acc[0][bx] = mma(g[0], xv[1], acc[0][bx]);
+                    acc[0][bx] = mma(g[1], xv[0], acc[0][bx]);
+                    acc[0][bx] = mma(g0s, xv[0], acc[0][bx]);
                 }
             }
         }
         if ((s % kPromote) == kPromote - 1 || s + 1 == stages) {
 #pragma unroll
             for (int bx = 0; bx < NBX; ++bx) {
-                if constexpr (A == 2) {
-                    main_acc[bx] += acc[0][bx] + acc[1][bx] * (1.f / kLoScale);
+                if constexpr (S == kF16x2s) {
+                    main_acc[bx] += acc[0][bx] * (1.f / kLoScale);
                 } else {
                     main_acc[bx] += acc[0][bx];
                 }
@@ -556,6 +562,20 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
     return true;
 }
 
+double mfma_zscale(int kernel, double amp_max)
+{
+    double z = 1.0;
+    if (scheme_of(kernel) != kF16x2s) return z;
+    // SCARF_FP16_AMP_LIMIT lowers the limit (tests reach the rescaled path)
+    double limit = 8.0;
+    if (const char *e = std::getenv("SCARF_FP16_AMP_LIMIT")) {
+        const double v = std::atof(e);
+        if (v > 0.0 && v < 8.0) limit = v;
+    }
+    for (int i = 0; i < 2000 && amp_max * z >= limit; ++i) z *= 0.5;
+    return z;
+}
+
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err)
 {
@@ -567,17 +587,19 @@ bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, 
     switch (scheme_of(plan.kernel)) {
     case kBf16x3:
         mfma_operand_tables<kBf16x3><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds, reinterpret_cast<__bf16 *>(base),
+                                                                  g.oy, g.oz, g.ds, plan.zscale,
+                                                                  reinterpret_cast<__bf16 *>(base),
                                                                   yt, zt);
         break;
     case kBf16x2:
         mfma_operand_tables<kBf16x2><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds, reinterpret_cast<__bf16 *>(base),
+                                                                  g.oy, g.oz, g.ds, plan.zscale,
+                                                                  reinterpret_cast<__bf16 *>(base),
                                                                   yt, zt);
         break;
     default:
         mfma_operand_tables<kF16x2s><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds,
+                                                                  g.oy, g.oz, g.ds, plan.zscale,
                                                                   reinterpret_cast<_Float16 *>(base), yt, zt);
         break;
     }
@@ -617,6 +639,7 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
                 double *d_part, hipStream_t stream, std::string &err)
 {
     const char *base = static_cast<const char *>(d_tab);
+    scaling /= plan.zscale;  // exact: a power of two
     switch (scheme_of(plan.kernel)) {
     case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
     case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
@@ -692,6 +715,7 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
             return false;
         }
         if (!mfma_plan(e.g, e.np_, kernel, plans[m], err, e.nbx)) return false;
+        plans[m].zscale = mfma_zscale(kernel, e.amp_max);
         rec_off[m] = rec_total;
         rec_total += size_t(5) * e.np_;
         tab_off[m] = tab_total;
@@ -707,6 +731,8 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         std::memcpy(&bits, &ds, sizeof bits);
         mix(bits);
         std::memcpy(&bits, &sg, sizeof bits);
+        mix(bits);
+        std::memcpy(&bits, &plans[m].zscale, sizeof bits);  // baked into the captured table kernel
         mix(bits);
     }
 
@@ -757,7 +783,7 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
             d.tk = p.tk;
             d.nbx = p.nbx;
             d.part_base = part_base[m];
-            d.scaling = members[m].scaling;
+            d.scaling = members[m].scaling / p.zscale;
             d.sigma = members[m].sigma;
         }
         ENS_TRY(hipMemcpy(d_desc_, desc_host_.data(), desc_host_.size(), hipMemcpyHostToDevice));
@@ -807,11 +833,13 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         // scaling can differ between runs of the same shape (sigma, spectrum)
         MemberDesc *desc = reinterpret_cast<MemberDesc *>(desc_host_.data());
         bool changed = false;
-        for (int m = 0; m < n; ++m)
-            if (desc[m].scaling != members[m].scaling) {
-                desc[m].scaling = members[m].scaling;
+        for (int m = 0; m < n; ++m) {
+            const double sc = members[m].scaling / plans[m].zscale;
+            if (desc[m].scaling != sc) {
+                desc[m].scaling = sc;
                 changed = true;
             }
+        }
         if (changed) ENS_TRY(hipMemcpy(d_desc_, desc_host_.data(), desc_host_.size(), hipMemcpyHostToDevice));
     }
     for (int m = 0; m < n; ++m)

@@ -33,7 +33,12 @@ struct MfmaPlan {
     long blocks = 0;
     int nparts = 0;         // statistics partials (one per block)
     size_t x_bytes = 0, y_bytes = 0, z_bytes = 0, table_bytes = 0;
+    double zscale = 1.0;    // power of two applied to Z and undone in the scaling (fp16 range)
 };
+
+// Power of two that keeps |Z Y| * 2^12 inside fp16 for the scaled-fp16
+// scheme (1 for the bf16 schemes and whenever the largest amplitude is < 8).
+double mfma_zscale(int kernel, double amp_max);
 
 // nbx = 2..4 forces the x tile width (tuner); other values pick it by shape
 bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0);
@@ -53,6 +58,7 @@ struct EnsembleMember {
     double sigma = 0;
     void *d_field = nullptr;  // device output, `Real` elements
     int nbx = 0;              // x tile override (0: by shape)
+    double amp_max = 0;       // largest harmonic amplitude of the record
 };
 
 // Monte Carlo ensembles: operand tables of every member, one persistent

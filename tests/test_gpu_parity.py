@@ -155,3 +155,18 @@ def test_same_seed_same_field_different_seed_differs(lib_f):
     c, _ = _run(lib_f, cfg, 12, "mfma_bf16x3")
     assert np.array_equal(a, b)
     assert not np.allclose(a, c)
+
+
+def test_scaled_fp16_amplitude_headroom(oracle, lib_f, monkeypatch):
+    """The scaled-fp16 scheme multiplies the generated operand's high part by
+    2^12, which needs |Z Y| < 8; larger amplitudes scale Z down by a power of
+    two and the field's scaling up by the same. The limit is lowered here so
+    that the rescaled path runs on an ordinary draw."""
+    cfg = structured_cfg(n=(96, 40, 12), nharm=256, real=np.float32)
+    ref, _ = oracle_field(oracle, cfg, 31, "f")
+    monkeypatch.setenv("SCARF_FP16_AMP_LIMIT", "0.125")
+    _say("mfma_f16x2 with amplitudes rescaled by 2^-k")
+    field, _ = _run(lib_f, cfg, 31, "mfma_f16x2")
+    err = np.abs(field.astype(np.float64) - ref.reshape(field.shape, order="F")).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}", flush=True)
+    assert err <= TOL["mfma_f16x2"]
