@@ -340,6 +340,7 @@ def main(argv=None) -> int:
     log(f"[bench] {args.workload}: {shape} x {w['nharm']} harmonics ({n_used} kept), host draw {host_s:.3f} s")
 
     res = {kernel: time_kernel(lib, hip, kernel, d_field, args.steps, args.warmup)}
+    tile = lib.last_tile() if kernel.startswith("mfma") else None
     main_field = d_field.numpy()
     log(f"[bench] {kernel}: {res[kernel]['field_ms']:.3f} ms field kernel, {res[kernel]['ms_per_step']:.3f} ms/step")
 
@@ -425,6 +426,8 @@ def main(argv=None) -> int:
               "acf": "von Karman" if w["acf"] == 0 else "Gaussian", "cl_m": list(w["cl"]), "sigma": w["sigma"],
               "seed": w["seed"], "kernel": kernel, "library": "libscarf3df" if flavour == "f" else "libscarf3d",
               "host_draw_s": _sig(host_s)}
+    if tile:
+        config["block_tile"] = dict(zip(("x", "j", "k"), tile))
     if structured:
         config["grid"] = list(w["n"])
     else:

@@ -83,6 +83,14 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
 /* Device milliseconds of the last ensemble (graph launch to completion). */
 double scarf_ensemble_last_ms(void);
 
+/* Block tile of the last MFMA run: out = {x points, j values, k values}
+ * (j x k = 256 grid columns). The tile is picked per grid shape by a tuner
+ * that times the candidates on a short synthetic harmonic record (SCARF_TUNE=0:
+ * least padding instead; SCARF_TUNE_CACHE=<file> keeps the choices across
+ * processes); it changes the speed only, never the field. Returns 3, or 0
+ * before any MFMA run. */
+int scarf_last_tile(int out[3]);
+
 /* Device milliseconds of the last run: {tables, field, statistics,
  * epilogue, total}; returns the number written (<= n). */
 int scarf_last_timings(double *ms, int n);

@@ -30,7 +30,7 @@ EXPORTS = (
     "scarf_set_nharm", "scarf_get_nharm", "scarf_set_kernel", "scarf_get_kernel", "scarf_last_error",
     "scarf_execute_device", "scarf_srm_draw", "scarf_srm_run", "scarf_last_timings", "scarf_stream",
     "scarf_setup_values", "scarf_draw_harmonics", "scarf_get_config", "scarf_set_option", "scarf_srm_run_fp64",
-    "scarf_ensemble_execute", "scarf_ensemble_last_ms",
+    "scarf_ensemble_execute", "scarf_ensemble_last_ms", "scarf_last_tile",
 )
 
 
@@ -131,6 +131,8 @@ class Library:
         L.scarf_set_option.restype = C.c_int
         L.scarf_srm_run_fp64.argtypes = [C.c_void_p, P(C.c_double)]
         L.scarf_srm_run_fp64.restype = C.c_int
+        L.scarf_last_tile.argtypes = [P(C.c_int)]
+        L.scarf_last_tile.restype = C.c_int
 
     def _arr(self, a, dtype=None):
         a = np.ascontiguousarray(a, dtype=dtype or self.dtype)
@@ -292,6 +294,11 @@ class Library:
 
     def ensemble_ms(self) -> float:
         return self.lib.scarf_ensemble_last_ms()
+
+    def last_tile(self):
+        """(x points, j values, k values) of the last MFMA run's block tile, or None."""
+        out = (C.c_int * 3)()
+        return tuple(out) if self.lib.scarf_last_tile(out) == 3 else None
 
     def set_option(self, name: str, value: float) -> int:
         return self.lib.scarf_set_option(name.encode(), float(value))

@@ -600,6 +600,18 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
         em[m].sigma = cfgs[m].sigma;
         em[m].d_field = d_fields[m];
     }
+    // tiles per member shape (tuned once per shape and kept)
+    if (scarf::TileTuner::enabled()) {
+        std::string terr;
+        for (int m = 0; m < n; ++m) {
+            scarf::TileChoice tc;
+            if (!s.device->tuner().choose<real>(em[m].g, kernel, static_cast<real *>(d_fields[m]),
+                                                s.device->stream(), tc, terr))
+                return fail("tile tuner: " + terr);
+            em[m].nbx = tc.nbx;
+            em[m].tk = tc.tk;
+        }
+    }
     std::vector<double> st;
     double ms = 0;
     std::string err;
@@ -622,6 +634,18 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
 }
 
 double scarf_ensemble_last_ms(void) { return state().ensemble_ms; }
+
+int scarf_last_tile(int out[3])
+{
+    State &s = state();
+    if (!out) return fail("scarf_last_tile: out is NULL");
+    out[0] = out[1] = out[2] = 0;
+    if (!s.device || s.device->last_nbx() == 0) return 0;
+    out[0] = 32 * s.device->last_nbx();
+    out[2] = s.device->last_tk();
+    out[1] = 256 / out[2];
+    return 3;
+}
 
 int scarf_srm_run_fp64(double *d_field, double stats_out[4])
 {

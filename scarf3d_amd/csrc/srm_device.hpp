@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,8 @@ struct Timings {
     double tables = 0, field = 0, stats = 0, epilogue = 0, total = 0;
 };
 
+class TileTuner;
+
 class Device {
 public:
     Device() = default;
@@ -78,6 +81,12 @@ public:
 
     int nharm_resident() const { return nharm_; }
 
+    // tile autotuner of the MFMA kernels (shared with ensembles)
+    TileTuner &tuner();
+    // x-tile width (32 * nbx) and column tile tk of the last MFMA run
+    int last_nbx() const { return last_nbx_; }
+    int last_tk() const { return last_tk_; }
+
 private:
     bool reserve(void **ptr, size_t &cap, size_t bytes, std::string &err);
 
@@ -86,6 +95,8 @@ private:
     hipEvent_t ev_[6] = {};
     int nharm_ = 0;
     double amp_max_ = 0;  // largest amplitude of the resident table
+    std::unique_ptr<TileTuner> tuner_;
+    int last_nbx_ = 0, last_tk_ = 0;
     // harmonic record: wx, wy, wz (rad/m), phase (rad), amp; plus revolution copies
     double *d_harm_ = nullptr;
     size_t harm_cap_ = 0;

@@ -449,6 +449,12 @@ bool Epilogue::active() const
         }                                                                                    \
     } while (0)
 
+TileTuner &Device::tuner()
+{
+    if (!tuner_) tuner_.reset(new TileTuner);
+    return *tuner_;
+}
+
 Device::~Device()
 {
     if (device_ < 0) return;
@@ -560,15 +566,20 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
         err = "empty field";
         return false;
     }
+    const bool mfma = g.structured && (kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 ||
+                                       kernel == kKernelMfmaBf16x2);
+    TileChoice tile;
+    if (mfma && TileTuner::enabled() && !tuner().choose<Real>(g, kernel, d_field, stream_, tile, err)) return false;
     timings_ = Timings{};
     HIP_TRY(hipEventRecord(ev_[0], stream_));
 
     int nparts = 0;
     if (g.structured) {
-        const bool mfma = kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 || kernel == kKernelMfmaBf16x2;
         if (mfma) {
             MfmaPlan plan;
-            if (!mfma_plan(g, np_, kernel, plan, err)) return false;
+            if (!mfma_plan(g, np_, kernel, plan, err, tile.nbx, tile.tk)) return false;
+            last_nbx_ = plan.nbx;
+            last_tk_ = plan.tk;
             plan.zscale = mfma_zscale(kernel, amp_max_);
             void *p = d_tab_;
             if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;

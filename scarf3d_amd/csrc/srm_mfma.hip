@@ -500,7 +500,7 @@ int scheme_of(int kernel)
 
 }  // namespace
 
-bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx)
+bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx, int tk_force)
 {
     const int s = scheme_of(kernel);
     if (s < 0) {
@@ -536,6 +536,7 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
     // padded columns, ties going to more k (fewer distinct Y rows per stage)
     long best_waste = -1;
     for (int tk = 8; tk >= 1; tk /= 2) {
+        if ((tk_force == 1 || tk_force == 2 || tk_force == 4 || tk_force == 8) && tk != tk_force) continue;
         const int tj = kTileC / tk;
         const long waste = long((g.ny + tj - 1) / tj) * tj * long((g.nz + tk - 1) / tk) * tk - long(g.ny) * g.nz;
         if (best_waste < 0 || waste < best_waste) {
@@ -714,7 +715,7 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
             err = "ensemble members must be structured grids";
             return false;
         }
-        if (!mfma_plan(e.g, e.np_, kernel, plans[m], err, e.nbx)) return false;
+        if (!mfma_plan(e.g, e.np_, kernel, plans[m], err, e.nbx, e.tk)) return false;
         plans[m].zscale = mfma_zscale(kernel, e.amp_max);
         rec_off[m] = rec_total;
         rec_total += size_t(5) * e.np_;
@@ -723,7 +724,8 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         part_base[m] = parts_total;
         parts_total += plans[m].nparts;
         const size_t vals[] = {size_t(e.g.nx), size_t(e.g.ny), size_t(e.g.nz), size_t(e.g.ox), size_t(e.g.oy),
-                               size_t(e.g.oz), size_t(e.np_), size_t(plans[m].nbx), size_t(e.rescale),
+                               size_t(e.g.oz), size_t(e.np_), size_t(plans[m].nbx), size_t(plans[m].tk),
+                               size_t(e.rescale),
                                reinterpret_cast<size_t>(e.d_field)};
         for (size_t v : vals) mix(v);
         double ds = e.g.ds, sg = e.sigma;
@@ -861,6 +863,158 @@ template bool Ensemble::run<float>(hipStream_t, int, const std::vector<EnsembleM
                                    double &, std::string &);
 template bool Ensemble::run<double>(hipStream_t, int, const std::vector<EnsembleMember> &, std::vector<double> &,
                                     double &, std::string &);
+// ---------------------------------------------------------------------------
+// Tile autotuner
+namespace {
+
+constexpr int kTuneStages = 24;  // stages timed per candidate (384 harmonics)
+
+// synthetic record: distinct small wavenumbers, unit amplitudes
+__global__ void tune_record(double *rec, int np_)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= np_) return;
+    rec[n] = 1.0e-3 * (1 + n % 7);
+    rec[np_ + n] = 2.0e-3 * (1 + n % 5);
+    rec[2 * np_ + n] = 3.0e-3 * (1 + n % 3);
+    rec[3 * np_ + n] = 0.1 * n;
+    rec[4 * np_ + n] = 1.0;
+}
+
+bool tune_reserve(void **p, size_t &cap, size_t bytes, std::string &err)
+{
+    if (bytes <= cap && *p) return true;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        err = std::string("tile tuner: hipMalloc: ") + hipGetErrorString(e);
+        return false;
+    }
+    cap = bytes;
+    return true;
+}
+
+}  // namespace
+
+TileTuner::~TileTuner()
+{
+    if (d_rec_) (void)hipFree(d_rec_);
+    if (d_tab_) (void)hipFree(d_tab_);
+    if (d_part_) (void)hipFree(d_part_);
+    for (hipEvent_t &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+bool TileTuner::enabled()
+{
+    const char *e = std::getenv("SCARF_TUNE");
+    return !(e && e[0] == '0');
+}
+
+void TileTuner::load_disk()
+{
+    if (disk_loaded_) return;
+    disk_loaded_ = true;
+    const char *path = std::getenv("SCARF_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    long k[5];
+    int nbx, tk;
+    while (std::fscanf(f, "%ld %ld %ld %ld %ld %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &nbx, &tk) == 7) {
+        if (nbx < 2 || nbx > 4 || (tk != 1 && tk != 2 && tk != 4 && tk != 8)) continue;
+        TileChoice c;
+        c.nbx = nbx;
+        c.tk = tk;
+        c.tuned = true;
+        cache_[Key{k[0], k[1], k[2], k[3], k[4]}] = c;
+    }
+    std::fclose(f);
+}
+
+void TileTuner::save_disk(const Key &k, const TileChoice &c)
+{
+    const char *path = std::getenv("SCARF_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%ld %ld %ld %ld %ld %d %d\n", k[0], k[1], k[2], k[3], k[4], c.nbx, c.tk);
+    std::fclose(f);
+}
+
+template <class Real>
+bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream_t stream, TileChoice &out,
+                       std::string &err)
+{
+    const Key key{long(kernel), long(sizeof(Real)), long(g.nx), long(g.ny), long(g.nz)};
+    load_disk();
+    const auto it = cache_.find(key);
+    if (it != cache_.end()) {
+        out = it->second;
+        return true;
+    }
+    const int np_ = kTuneStages * kStageH;
+    MfmaPlan def;
+    if (!mfma_plan(g, np_, kernel, def, err)) return false;
+    if (!d_rec_) {
+        ENS_TRY(hipMalloc(&d_rec_, size_t(5) * np_ * sizeof(double)));
+        ENS_TRY(hipEventCreate(&ev_[0]));
+        ENS_TRY(hipEventCreate(&ev_[1]));
+        tune_record<<<(np_ + 255) / 256, 256, 0, stream>>>(d_rec_, np_);
+        ENS_TRY(hipGetLastError());
+    }
+    auto padded = [&](const MfmaPlan &p) { return double(p.blocks) * kTileC * 32.0 * p.nbx; };
+    TileChoice best;
+    double best_ms = -1.0, def_ms = -1.0;
+    for (int nbx = 4; nbx >= 2; --nbx) {
+        for (int tk = 8; tk >= 1; tk /= 2) {
+            MfmaPlan p;
+            if (!mfma_plan(g, np_, kernel, p, err, nbx, tk)) return false;
+            const bool is_def = p.nbx == def.nbx && p.tk == def.tk;
+            if (!is_def && padded(p) > 2.0 * padded(def)) continue;  // far more padding: cannot win
+            if (!tune_reserve(&d_tab_, tab_cap_, p.table_bytes, err)) return false;
+            if (!tune_reserve(&d_part_, part_cap_, size_t(3) * p.nparts * sizeof(double), err)) return false;
+            if (!mfma_tables(p, g, d_rec_, np_, d_tab_, stream, err)) return false;
+            if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err)) return false;
+            float t_min = -1.f;
+            for (int rep = 0; rep < 3; ++rep) {
+                ENS_TRY(hipEventRecord(ev_[0], stream));
+                if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err))
+                    return false;
+                ENS_TRY(hipEventRecord(ev_[1], stream));
+                ENS_TRY(hipEventSynchronize(ev_[1]));
+                float ms = 0.f;
+                ENS_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+                if (t_min < 0.f || ms < t_min) t_min = ms;
+            }
+            if (is_def) def_ms = t_min;
+            if (best_ms < 0.0 || t_min < best_ms) {
+                best_ms = t_min;
+                best.nbx = p.nbx;
+                best.tk = p.tk;
+            }
+        }
+    }
+    // keep the least-padding tile unless another is clearly faster (timing noise)
+    if (def_ms > 0.0 && best_ms > 0.97 * def_ms) {
+        best.nbx = def.nbx;
+        best.tk = def.tk;
+        best_ms = def_ms;
+    }
+    best.ms = best_ms;
+    best.ms_default = def_ms;
+    best.tuned = true;
+    cache_[key] = best;
+    save_disk(key, best);
+    out = best;
+    return true;
+}
+
+template bool TileTuner::choose<float>(const Geometry &, int, float *, hipStream_t, TileChoice &, std::string &);
+template bool TileTuner::choose<double>(const Geometry &, int, double *, hipStream_t, TileChoice &, std::string &);
+
 template bool mfma_field<double>(const MfmaPlan &, const Geometry &, const void *, double, double *, double *,
                                  hipStream_t, std::string &);
 

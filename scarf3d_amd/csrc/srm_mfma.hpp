@@ -12,6 +12,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -40,8 +42,9 @@ struct MfmaPlan {
 // scheme (1 for the bf16 schemes and whenever the largest amplitude is < 8).
 double mfma_zscale(int kernel, double amp_max);
 
-// nbx = 2..4 forces the x tile width (tuner); other values pick it by shape
-bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0);
+// nbx = 2..4 forces the x tile width, tk = 1, 2, 4 or 8 the column tile
+// (tj = 256 / tk); other values pick them by shape (least padding)
+bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0, int tk = 0);
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err);
 template <class Real>
@@ -58,7 +61,48 @@ struct EnsembleMember {
     double sigma = 0;
     void *d_field = nullptr;  // device output, `Real` elements
     int nbx = 0;              // x tile override (0: by shape)
+    int tk = 0;               // column tile override (0: by shape)
     double amp_max = 0;       // largest harmonic amplitude of the record
+};
+
+// Tile autotuner. For a grid and an MFMA kernel it times every candidate
+// block tile (x width 64/96/128 by column shape tj x tk, tj * tk = 256) on a
+// short synthetic harmonic record (the kernels' speed does not depend on
+// the values) and keeps the fastest. The tile never changes the field: each
+// output value sums the same products in the same order. Choices are cached
+// per (kernel, output type, nx, ny, nz) for the process and, when
+// SCARF_TUNE_CACHE names a file, across processes. SCARF_TUNE=0 turns tuning
+// off (tiles by least padding).
+struct TileChoice {
+    int nbx = 0, tk = 0;
+    double ms = 0;        // time of the timed stages with this tile
+    double ms_default = 0;  // the same for the least-padding tile
+    bool tuned = false;
+};
+
+class TileTuner {
+public:
+    TileTuner() = default;
+    ~TileTuner();
+    TileTuner(const TileTuner &) = delete;
+    TileTuner &operator=(const TileTuner &) = delete;
+
+    static bool enabled();
+    // d_scratch: device room for g.points() values of Real (overwritten)
+    template <class Real>
+    bool choose(const Geometry &g, int kernel, Real *d_scratch, hipStream_t stream, TileChoice &out, std::string &err);
+    int tuned_shapes() const { return static_cast<int>(cache_.size()); }
+
+private:
+    using Key = std::array<long, 5>;
+    void load_disk();
+    void save_disk(const Key &k, const TileChoice &c);
+    std::map<Key, TileChoice> cache_;
+    bool disk_loaded_ = false;
+    double *d_rec_ = nullptr;
+    void *d_tab_ = nullptr, *d_part_ = nullptr;
+    size_t tab_cap_ = 0, part_cap_ = 0;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
 };
 
 // Monte Carlo ensembles: operand tables of every member, one persistent

@@ -170,3 +170,25 @@ def test_scaled_fp16_amplitude_headroom(oracle, lib_f, monkeypatch):
     err = np.abs(field.astype(np.float64) - ref.reshape(field.shape, order="F")).max() / cfg["sigma"]
     print(f"  max|err|/sigma = {err:.3e}", flush=True)
     assert err <= TOL["mfma_f16x2"]
+
+
+def test_tile_tuner_changes_speed_not_field(oracle, lib_f, monkeypatch, tmp_path):
+    """Tuned and least-padding tiles sum the same products in the same order
+    per output value: the fields agree (to 1e-7 sigma; the same bits where
+    the hardware's per-element MFMA order does not depend on the tile). The
+    choice persists through SCARF_TUNE_CACHE."""
+    cache = tmp_path / "tiles.txt"
+    monkeypatch.setenv("SCARF_TUNE_CACHE", str(cache))
+    cfg = structured_cfg(n=(200, 70, 9), nharm=256, real=np.float32)
+    _say("tuned tile", cfg["fe"])
+    tuned, _ = _run(lib_f, cfg, 3, "mfma_bf16x3")
+    tile = lib_f.last_tile()
+    assert tile is not None and tile[1] * tile[2] == 256 and tile[0] in (64, 96, 128)
+    lines = cache.read_text().split("\n")
+    assert any(line.split()[2:5] == ["200", "70", "9"] for line in lines if line)
+    lib_f.finalize()
+    monkeypatch.setenv("SCARF_TUNE", "0")
+    plain, _ = _run(lib_f, cfg, 3, "mfma_bf16x3")
+    print(f"  tuned tile {tile}, least-padding tile {lib_f.last_tile()}, "
+          f"bit-identical: {np.array_equal(tuned, plain)}", flush=True)
+    assert np.abs(tuned.astype(np.float64) - plain).max() <= 1e-7 * cfg["sigma"]
