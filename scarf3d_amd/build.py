@@ -96,6 +96,27 @@ def build_fortran(verbose: bool = False) -> list[Path]:
     return outs
 
 
+CXX = os.environ.get("CXX_SCARF", "g++")
+
+
+def build_standalone(verbose: bool = False) -> list[Path]:
+    """The standalone program (standalone/scarf3d_main.cpp) for both flavours:
+    host code only, linked against the library next to it."""
+    outs = []
+    src = HERE / "standalone" / "scarf3d_main.cpp"
+    for flav, defs, libname in (("f", [], "scarf3df"), ("d", ["-DDOUBLE_PREC"], "scarf3d")):
+        exe = LIB / f"scarf3d_standalone_{flav}"
+        cmd = [CXX, "-std=c++17", "-O2", "-Wall", *defs, "-I", str(ROOT / "include"), str(src), "-L", str(LIB),
+               f"-l{libname}", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"standalone build failed:\n{res.stdout}{res.stderr}")
+        outs.append(exe)
+    return outs
+
+
 def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
     """Compile every HIP translation unit for gfx950 and link both libraries."""
     BUILD.mkdir(exist_ok=True)
@@ -112,6 +133,7 @@ def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
         _link([*shared, obj], tmp, verbose)
         os.replace(tmp, out)
         outs.append(out)
+    outs.extend(build_standalone(verbose))
     outs.extend(build_fortran(verbose))
     return outs
 
