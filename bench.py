@@ -102,8 +102,9 @@ def bench_ensemble(args):
     bufs = [hip.empty(int(np.prod(m["fe"])), np.float32) for m in members]
     ptrs = [b.ptr for b in bufs]
     t0 = time.perf_counter()
+    log(f"[bench] ensemble of {len(members)}: first run (draws, tile tuning per shape, graph capture, run)")
     stats = lib.ensemble_execute(members, ptrs, args.kernel)
-    log(f"[bench] ensemble first run (draw + graph capture + run) {time.perf_counter() - t0:.2f} s")
+    log(f"[bench] ensemble first run {time.perf_counter() - t0:.2f} s")
     for _ in range(max(0, args.warmup - 1)):
         lib.ensemble_execute(members, ptrs, args.kernel)
     dev_ms, wall = [], []

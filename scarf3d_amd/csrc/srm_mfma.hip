@@ -386,8 +386,6 @@ struct MemberDesc {
     double scaling, sigma;
 };
 
-__global__ void reset_counter(int *c) { c[0] = 0; }
-
 // per-member merge of the tile triples, then the `real` roundings of
 // src/scarflib_srm.f90:880-899 (one block per member)
 template <class Real>
@@ -445,17 +443,18 @@ __global__ __launch_bounds__(256) void ensemble_rescale(Real *__restrict__ field
 template <int S, class Out>
 __global__ __launch_bounds__(kThreads, 1) void mfma_ensemble(const MemberDesc *__restrict__ members,
                                                              const int2 *__restrict__ items, int nitems,
-                                                             int *__restrict__ counter, double *__restrict__ part)
+                                                             double *__restrict__ part)
 {
     using E = typename SchemeOf<S>::elt;
     __shared__ StageLds<SchemeOf<S>::parts> lds0, lds1;
-    __shared__ int next;
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = next;
-        __syncthreads();
-        if (it >= nitems) break;  // every wave of the block sees the same value and leaves together
+    // Block b takes items b, b + G, b + 2G, ... (G blocks) of the list sorted
+    // by cost, largest first: each round hands every block one item of
+    // nearly the same cost. The loop runs on blockIdx/gridDim only, so it is
+    // uniform by construction. (A work counter claimed by thread 0 and
+    // broadcast through LDS was laid out by the compiler as a loop nest in
+    // which the other lanes re-entered the item loop and its barriers ahead
+    // of thread 0's atomic, and never saw the counter advance.)
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 item = items[it];  // (member, tile within the member, x fastest)
         const MemberDesc m = members[item.x];
         const int x0 = (item.y % m.tiles_x) * 32 * m.nbx;
@@ -666,10 +665,10 @@ void Ensemble::release()
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
     graph_ = nullptr;
-    void *bufs[] = {d_rec_, d_tab_, d_part_, d_desc_, d_items_, d_counter_, d_stat_};
+    void *bufs[] = {d_rec_, d_tab_, d_part_, d_desc_, d_items_, d_stat_};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    d_rec_ = d_tab_ = d_part_ = d_desc_ = d_items_ = d_counter_ = d_stat_ = nullptr;
+    d_rec_ = d_tab_ = d_part_ = d_desc_ = d_items_ = d_stat_ = nullptr;
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     ev_[0] = ev_[1] = nullptr;
@@ -757,7 +756,6 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         ENS_TRY(hipMalloc(&d_part_, size_t(3) * parts_total * sizeof(double)));
         ENS_TRY(hipMalloc(&d_desc_, size_t(n) * sizeof(MemberDesc)));
         ENS_TRY(hipMalloc(&d_items_, items.size() * sizeof(int2)));
-        ENS_TRY(hipMalloc(&d_counter_, 256));
         ENS_TRY(hipMalloc(&d_stat_, size_t(6) * n * sizeof(double)));
         rec_bytes_ = rec_total * sizeof(double);
         nitems_ = static_cast<int>(items.size());
@@ -808,15 +806,13 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
                 return false;
             }
         }
-        reset_counter<<<1, 1, 0, stream>>>(static_cast<int *>(d_counter_));
         const MemberDesc *dd = static_cast<const MemberDesc *>(d_desc_);
         const int2 *di = static_cast<const int2 *>(d_items_);
-        int *dc = static_cast<int *>(d_counter_);
         double *dp = static_cast<double *>(d_part_);
         switch (s) {
-        case kBf16x3: mfma_ensemble<kBf16x3, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
-        case kBf16x2: mfma_ensemble<kBf16x2, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
-        default: mfma_ensemble<kF16x2s, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dc, dp); break;
+        case kBf16x3: mfma_ensemble<kBf16x3, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dp); break;
+        case kBf16x2: mfma_ensemble<kBf16x2, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dp); break;
+        default: mfma_ensemble<kF16x2s, Real><<<persistent, kThreads, 0, stream>>>(dd, di, nitems_, dp); break;
         }
         ensemble_finalize<Real><<<n, 256, 0, stream>>>(dd, dp, static_cast<double *>(d_stat_));
         for (int m = 0; m < n; ++m) {

@@ -106,9 +106,9 @@ private:
 };
 
 // Monte Carlo ensembles: operand tables of every member, one persistent
-// grouped-GEMM launch over all (member, tile) work items pulled from an
-// atomic counter (largest members first), per-member statistics and
-// rescale. The launch sequence is captured once per ensemble shape into a
+// grouped-GEMM launch over all (member, tile) work items (sorted by cost,
+// largest first, and dealt to the blocks round-robin), per-member
+// statistics and rescale. The launch sequence is captured once per ensemble shape into a
 // hipGraph and replayed; only the harmonic records change between runs.
 class Ensemble {
 public:
@@ -133,7 +133,7 @@ private:
     hipGraphExec_t exec_ = nullptr;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     void *d_rec_ = nullptr, *d_tab_ = nullptr, *d_part_ = nullptr, *d_desc_ = nullptr, *d_items_ = nullptr;
-    void *d_counter_ = nullptr, *d_stat_ = nullptr;
+    void *d_stat_ = nullptr;
     size_t rec_bytes_ = 0;
     int nitems_ = 0, nmembers_ = 0;
     std::vector<unsigned char> desc_host_;
