@@ -35,7 +35,7 @@ def test_ensemble_members_equal_single_runs(oracle, lib_f, kernel):
     fields = [b.numpy() for b in bufs]
     # replay of the captured graph with the same shapes gives the same result
     stats2 = lib_f.ensemble_execute(MEMBERS, [b.ptr for b in bufs], kernel)
-    assert np.array_equal(stats, stats2)
+    assert np.array_equal(stats[:, :4], stats2[:, :4])  # columns 5-6 are host and device seconds
     for i, m in enumerate(MEMBERS):
         cfg = _cfg(m)
         init_product(lib_f, cfg)

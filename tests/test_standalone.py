@@ -84,6 +84,7 @@ def test_standalone_field_equals_library(tmp_path, lib_f):
     cfg = structured_cfg(n=(40, 36, 20), ds=50.0, cl=(2000.0,) * 3, sigma=0.05, hurst=0.25, nharm=10000,
                          poi=np.array([[500.0], [500.0], [500.0]]), mute=0.0, taper=300.0, real=np.float32)
     init_product(lib_f, cfg)
+    assert lib_f.set_kernel("auto") == 0  # the standalone's choice (the setting outlives finalize)
     assert lib_f.set_option("mute", 0.0) == 0  # present, as the standalone passes it (a C opt drops mute <= 0)
     ref, _ = lib_f.execute(1234)
     assert np.array_equal(field, ref.reshape(-1, order="F"))
