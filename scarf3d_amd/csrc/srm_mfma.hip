@@ -10,6 +10,9 @@
 //     of its own column and 8 K values and splits it into 16-bit parts
 //     directly in MFMA operand layout (row c = lane % 32,
 //     K = 8 (lane / 32) .. +7);
+//   - each wave reads all x chunks of a k half, then issues the products in
+//     product-major order, so that the NBX accumulators' dependent chains
+//     interleave on the matrix core;
 //   - one vmcnt wait and one barrier end each stage.
 // Accumulation: FP32 MFMA accumulators, promoted into a second FP32
 // accumulator every 128 harmonics (8 stages), so the rounding of the long
@@ -91,13 +94,68 @@ __device__ inline void split(float x, typename SchemeOf<S>::elt (&out)[SchemeOf<
     }
 }
 
+// The same splits on a (re, im) pair, one 32-bit word per part: the low half
+// holds re's part, the high half im's, which is how a harmonic's two K
+// values sit in an MFMA operand register. Each rounding is one packed
+// conversion; the rounded values come back to FP32 exactly by a shift or a
+// conversion.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2v __attribute__((ext_vector_type(2)));
+
+template <int S>
+__device__ inline uint32_t pack2(float a, float b)
+{
+    if constexpr (S == kF16x2s) {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, f16x2v));
+    } else {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2v));
+    }
+}
+template <int S>
+__device__ inline void unpack2(uint32_t w, float &a, float &b)
+{
+    if constexpr (S == kF16x2s) {
+        const f16x2v h = __builtin_bit_cast(f16x2v, w);
+        a = static_cast<float>(h[0]);
+        b = static_cast<float>(h[1]);
+    } else {
+        a = __builtin_bit_cast(float, w << 16);
+        b = __builtin_bit_cast(float, w & 0xffff0000u);
+    }
+}
+template <int S>
+__device__ inline void split2(float re, float im, uint32_t (&out)[SchemeOf<S>::parts])
+{
+    out[0] = pack2<S>(re, im);
+    float hr, hi;
+    unpack2<S>(out[0], hr, hi);
+    const float rr = re - hr, ri = im - hi;
+    if constexpr (S == kBf16x3) {
+        out[1] = pack2<S>(rr, ri);
+        float mr, mi;
+        unpack2<S>(out[1], mr, mi);
+        out[2] = pack2<S>(rr - mr, ri - mi);
+    } else if constexpr (S == kBf16x2) {
+        out[1] = pack2<S>(rr, ri);
+    } else {
+        out[1] = pack2<S>(rr * kLoScale, ri * kLoScale);
+    }
+}
+
 __device__ inline f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 __device__ inline f32x16 mma(f16x8 a, f16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
 
-// LDS position of 16-byte chunk q (K = 8q..8q+7) of row r: the chunk index
-// is XOR-swizzled with row bits so that 8 consecutive rows reading the same
-// chunk hit 8 distinct 16-byte bank groups.
-__device__ inline int chunk_slot(int r, int q) { return r * 4 + (q ^ ((r >> 1) & 3)); }
+// LDS position of 16-byte chunk q (K = 8q..8q+7) of row r (64 bytes per
+// row and part): the chunk index is XOR-permuted by row bits so that the
+// lanes one ds_read_b128 serves together, consecutive rows reading the same
+// chunk, hit distinct 16-byte bank groups: any 8 rows 8m..8m+7 fill 128
+// bytes of banks and any 16 rows 16m..16m+15 fill 256 bytes. (Permuting by
+// (r >> 1) & 3 alone covers the 8-row case only; with 16 lanes served per
+// cycle it left rows r and r + 8 on the same banks, a 2-way conflict on every
+// x-operand read.)
+__device__ inline int chunk_perm(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
+__device__ inline int chunk_slot(int r, int q) { return r * 4 + (q ^ chunk_perm(r)); }
 
 // ---------------------------------------------------------------------------
 // Operand tables, built once per realization:
@@ -172,14 +230,27 @@ struct StageLds {
     float2 z[kStageH * 8];
 };
 
-__device__ inline void lds_dma16(const void *src, void *lds_wave_base)
+// LDS DMA of 16 or 4 bytes per lane from base + off (base the same for the
+// whole wave, off a 32-bit per-lane byte offset) to lds_wave_base + lane * size
+typedef const __attribute__((address_space(1))) char *global_bytes;
+// base is uniform; taking it through readfirstlane keeps it in scalar
+// registers, so that the load uses the scalar-base + 32-bit lane offset form
+// instead of a 64-bit per-lane address
+__device__ inline global_bytes uniform_base(const char *base)
 {
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+    const uint64_t b = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+    return (global_bytes)((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ inline void lds_dma16(const char *base, uint32_t off, void *lds_wave_base)
+{
+    __builtin_amdgcn_global_load_lds(uniform_base(base) + off,
                                      (__attribute__((address_space(3))) void *)lds_wave_base, 16, 0, 0);
 }
-__device__ inline void lds_dma4(const void *src, void *lds_wave_base)
+__device__ inline void lds_dma4(const char *base, uint32_t off, void *lds_wave_base)
 {
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+    __builtin_amdgcn_global_load_lds(uniform_base(base) + off,
                                      (__attribute__((address_space(3))) void *)lds_wave_base, 4, 0, 0);
 }
 
@@ -209,40 +280,47 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
     constexpr int A = Sch::accs;
     constexpr int TX = 32 * NBX;
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: in a scalar register, the LDS targets of the
+    // DMAs (M0) are scalar arithmetic
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tj = 1 << tj_shift;
     const int tj_mask = tj - 1;
 
-    // x-operand chunk of this lane: LDS slot tid, global chunk swizzled back
+    // DMA sources: a per-lane byte offset fixed for the tile plus a base that
+    // is the same for the whole block (stage, part, y row group), so that the
+    // per-stage address work is scalar.
+    //   x: LDS slot tid holds the chunk the permutation puts there
     const bool xload = tid < TX * 4;
     const int xr = tid >> 2;
-    const int xq = (tid & 3) ^ ((xr >> 1) & 3);
-    const uint4 *xsrc = reinterpret_cast<const uint4 *>(xt) + (size_t(x0 + xr) * 4 + xq);
-    const size_t xpart_stride = size_t(nx_pad) * 4;  // uint4 per part of a stage
-    // y / z floats of this lane: index i = q * 512 + tid in the stage tile
+    const int xq = (tid & 3) ^ chunk_perm(xr);
+    const uint32_t xoff = uint32_t((x0 + xr) * 4 + xq) * 16u;
+    const size_t xplane = size_t(nx_pad) * 64;  // bytes of one part of one stage
+    //   y: float i = q * 512 + tid of the stage tile = row h, element e of
+    //      the tile row (2 * tj floats); 2 * tj divides 512, so h advances by
+    //      512 / (2 tj) rows per q and e is the same for every q
     const int ny_floats = 32 * tj, nz_floats = 32 * tk;
-    const float *yf = reinterpret_cast<const float *>(yt);
-    const float *zf = reinterpret_cast<const float *>(zt);
+    const int ye = tid & (2 * tj - 1), yh = tid >> (tj_shift + 1);
+    const uint32_t yoff = uint32_t((yh * ny + min(j0 + (ye >> 1), ny - 1)) * 2 + (ye & 1)) * 4u;
+    const size_t yq_step = size_t(kThreads / (2 * tj)) * ny * 8;  // bytes per q
+    const size_t ystage = size_t(kStageH) * ny * 8;
+    //   z: float tid of the stage tile (fewer than 512)
+    const int ze = tid % (2 * tk), zh = tid / (2 * tk);
+    const uint32_t zoff = uint32_t((zh * nz + min(k0 + (ze >> 1), nz - 1)) * 2 + (ze & 1)) * 4u;
+    const size_t zstage = size_t(kStageH) * nz * 8;
+    const char *xb = reinterpret_cast<const char *>(xt);
+    const char *yb = reinterpret_cast<const char *>(yt);
+    const char *zb = reinterpret_cast<const char *>(zt);
 
     auto issue_loads = [&](int s, StageLds<P> &dst) {
         if (xload) {
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                lds_dma16(xsrc + (size_t(s) * P + p) * xpart_stride, &dst.x[p][wave * 64]);
+                lds_dma16(xb + (size_t(s) * P + p) * xplane, xoff, &dst.x[p][wave * 64]);
         }
-        for (int i0 = 0; i0 < ny_floats; i0 += kThreads) {
-            const int i = i0 + tid;  // ny_floats is a multiple of 512
-            const int h = i / (2 * tj), e = i - h * 2 * tj;
-            const int j = min(j0 + (e >> 1), ny - 1);
-            lds_dma4(yf + (size_t(s * kStageH + h) * ny + j) * 2 + (e & 1),
-                     reinterpret_cast<float *>(dst.y) + i0 + wave * 64);
-        }
-        if (tid < nz_floats) {
-            const int h = tid / (2 * tk), e = tid - h * 2 * tk;
-            const int k = min(k0 + (e >> 1), nz - 1);
-            lds_dma4(zf + (size_t(s * kStageH + h) * nz + k) * 2 + (e & 1),
-                     reinterpret_cast<float *>(dst.z) + wave * 64);
-        }
+        const char *ys = yb + size_t(s) * ystage;
+        for (int q = 0; q * kThreads < ny_floats; ++q)
+            lds_dma4(ys + q * yq_step, yoff, reinterpret_cast<float *>(dst.y) + q * kThreads + wave * 64);
+        if (tid < nz_floats) lds_dma4(zb + size_t(s) * zstage, zoff, reinterpret_cast<float *>(dst.z) + wave * 64);
     };
 
     f32x16 acc[A][NBX], main_acc[NBX];
@@ -267,7 +345,7 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
             // generated operand: Z*Y for 4 harmonics x (re, im) of this lane's column
-            E gp[P][8];
+            uint32_t gw[P][4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int h = kk * 8 + khalf * 4 + t;
@@ -275,44 +353,43 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
                 const float2 z = cur.z[h * tk + ck];
                 const float re = z.x * y.x - z.y * y.y;
                 const float im = z.x * y.y + z.y * y.x;
-                E pr[P], pi[P];
-                split<S>(re, pr);
-                split<S>(im, pi);
+                uint32_t w[P];
+                split2<S>(re, im, w);
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    gp[p][2 * t] = pr[p];
-                    gp[p][2 * t + 1] = pi[p];
-                }
+                for (int p = 0; p < P; ++p) gw[p][t] = w[p];
             }
             V g[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) g[p][e] = gp[p][e];
+            for (int p = 0; p < P; ++p) g[p] = __builtin_bit_cast(V, uint4{gw[p][0], gw[p][1], gw[p][2], gw[p][3]});
             V g0s{};
             if constexpr (S == kF16x2s) g0s = g[0] * static_cast<E>(kLoScale);  // exact: power of two, |G| < 8
+            // all x chunks of this k half first, then the products in
+            // product-major order: the NBX accumulators' chains interleave
+            // (each accumulator still sees its products in the same order)
+            V xv[NBX][P];
 #pragma unroll
             for (int bx = 0; bx < NBX; ++bx) {
-                const int row = bx * 32 + (lane & 31);
-                const int slot = chunk_slot(row, kk * 2 + khalf);
-                V xv[P];
+                const int slot = chunk_slot(bx * 32 + (lane & 31), kk * 2 + khalf);
 #pragma unroll
-                for (int p = 0; p < P; ++p) xv[p] = __builtin_bit_cast(V, cur.x[p][slot]);
-                if constexpr (S == kBf16x3) {
-                    acc[0][bx] = mma(g[1], xv[1], acc[0][bx]);
-                    acc[0][bx] = mma(g[0], xv[2], acc[0][bx]);
-                    acc[0][bx] = mma(g[2], xv[0], acc[0][bx]);
-                    acc[0][bx] = mma(g[0], xv[1], acc[0][bx]);
-                    acc[0][bx] = mma(g[1], xv[0], acc[0][bx]);
-                    acc[0][bx] = mma(g[0], xv[0], acc[0][bx]);
-                } else if constexpr (S == kBf16x2) {
-                    acc[0][bx] = mma(g[0], xv[1], acc[0][bx]);
-                    acc[0][bx] = mma(g[1], xv[0], acc[0][bx]);
-                    acc[0][bx] = mma(g[0], xv[0], acc[0][bx]);
-                } else {
-                    acc[0][bx] = mma(g[0], xv[1], acc[0][bx]);
-                    acc[0][bx] = mma(g[1], xv[0], acc[0][bx]);
-                    acc[0][bx] = mma(g0s, xv[0], acc[0][bx]);
+                for (int p = 0; p < P; ++p) xv[bx][p] = __builtin_bit_cast(V, cur.x[p][slot]);
+            }
+            constexpr int NP = S == kBf16x3 ? 6 : 3;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+#pragma unroll
+                for (int bx = 0; bx < NBX; ++bx) {
+                    f32x16 &a = acc[0][bx];
+                    if constexpr (S == kBf16x3) {
+                        constexpr int gi[6] = {1, 0, 2, 0, 1, 0}, xi[6] = {1, 2, 0, 1, 0, 0};
+                        a = mma(g[gi[q]], xv[bx][xi[q]], a);
+                    } else if constexpr (S == kBf16x2) {
+                        constexpr int gi[3] = {0, 1, 0}, xi[3] = {1, 0, 0};
+                        a = mma(g[gi[q]], xv[bx][xi[q]], a);
+                    } else {
+                        if (q == 0) a = mma(g[0], xv[bx][1], a);
+                        else if (q == 1) a = mma(g[1], xv[bx][0], a);
+                        else a = mma(g0s, xv[bx][0], a);
+                    }
                 }
             }
         }
