@@ -600,13 +600,13 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
         em[m].sigma = cfgs[m].sigma;
         em[m].d_field = d_fields[m];
     }
-    // tiles per member shape (tuned once per shape and kept)
+    // tiles per member: in one full launch a member's tile is worth its
+    // padded work over the tile's saturated throughput (timed once)
     if (scarf::TileTuner::enabled()) {
         std::string terr;
         for (int m = 0; m < n; ++m) {
             scarf::TileChoice tc;
-            if (!s.device->tuner().choose<real>(em[m].g, kernel, static_cast<real *>(d_fields[m]),
-                                                s.device->stream(), tc, terr))
+            if (!s.device->tuner().choose_saturated<real>(em[m].g, kernel, s.device->stream(), tc, terr))
                 return fail("tile tuner: " + terr);
             em[m].nbx = tc.nbx;
             em[m].tk = tc.tk;

@@ -973,6 +973,7 @@ bool tune_reserve(void **p, size_t &cap, size_t bytes, std::string &err)
 
 TileTuner::~TileTuner()
 {
+    if (d_sat_) (void)hipFree(d_sat_);
     if (d_rec_) (void)hipFree(d_rec_);
     if (d_tab_) (void)hipFree(d_tab_);
     if (d_part_) (void)hipFree(d_part_);
@@ -1085,6 +1086,78 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
     return true;
 }
 
+template <class Real>
+bool TileTuner::choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err)
+{
+    const int np_ = kTuneStages * kStageH;
+    const long ks = long(sizeof(Real));
+    // the throughput table of this kernel and output type, timed once
+    if (rate_.find({long(kernel), ks, 4L, 8L}) == rate_.end()) {
+        Geometry sg;  // 384 x 512 x 128: no x padding for 64/96/128, no column padding for any tj x tk
+        sg.structured = true;
+        sg.nx = 384;
+        sg.ny = 512;
+        sg.nz = 128;
+        sg.ds = 10.0;
+        if (!tune_reserve(&d_sat_, sat_cap_, size_t(sg.points()) * sizeof(Real), err)) return false;
+        if (!d_rec_) {
+            ENS_TRY(hipMalloc(&d_rec_, size_t(5) * np_ * sizeof(double)));
+            ENS_TRY(hipEventCreate(&ev_[0]));
+            ENS_TRY(hipEventCreate(&ev_[1]));
+            tune_record<<<(np_ + 255) / 256, 256, 0, stream>>>(d_rec_, np_);
+            ENS_TRY(hipGetLastError());
+        }
+        Real *scratch = static_cast<Real *>(d_sat_);
+        for (int nbx = 4; nbx >= 2; --nbx) {
+            for (int tk = 8; tk >= 1; tk /= 2) {
+                MfmaPlan p;
+                if (!mfma_plan(sg, np_, kernel, p, err, nbx, tk)) return false;
+                if (p.nbx != nbx || p.tk != tk) continue;
+                if (!tune_reserve(&d_tab_, tab_cap_, p.table_bytes, err)) return false;
+                if (!tune_reserve(&d_part_, part_cap_, size_t(3) * p.nparts * sizeof(double), err)) return false;
+                if (!mfma_tables(p, sg, d_rec_, np_, d_tab_, stream, err)) return false;
+                if (!mfma_field(p, sg, d_tab_, 1.0, scratch, static_cast<double *>(d_part_), stream, err)) return false;
+                float t_min = -1.f;
+                for (int rep = 0; rep < 3; ++rep) {
+                    ENS_TRY(hipEventRecord(ev_[0], stream));
+                    if (!mfma_field(p, sg, d_tab_, 1.0, scratch, static_cast<double *>(d_part_), stream, err))
+                        return false;
+                    ENS_TRY(hipEventRecord(ev_[1], stream));
+                    ENS_TRY(hipEventSynchronize(ev_[1]));
+                    float ms = 0.f;
+                    ENS_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+                    if (t_min < 0.f || ms < t_min) t_min = ms;
+                }
+                rate_[{long(kernel), ks, long(nbx), long(tk)}] = double(sg.points()) * np_ / std::max(t_min, 1e-6f);
+            }
+        }
+    }
+    // the member's candidates: padded work / saturated rate
+    double best = -1.0;
+    for (int nbx = 4; nbx >= 2; --nbx) {
+        for (int tk = 8; tk >= 1; tk /= 2) {
+            const auto it = rate_.find({long(kernel), ks, long(nbx), long(tk)});
+            if (it == rate_.end()) continue;
+            MfmaPlan p;
+            if (!mfma_plan(g, np_, kernel, p, err, nbx, tk)) return false;
+            if (p.nbx != nbx || p.tk != tk) continue;
+            const double padded = double(p.blocks) * kTileC * 32.0 * nbx;  // points incl. padding
+            const double cost = padded / it->second;
+            if (best < 0.0 || cost < best) {
+                best = cost;
+                out.nbx = nbx;
+                out.tk = tk;
+            }
+        }
+    }
+    out.ms = best * np_;
+    out.ms_default = 0.0;
+    out.tuned = true;
+    return best >= 0.0;
+}
+
+template bool TileTuner::choose_saturated<float>(const Geometry &, int, hipStream_t, TileChoice &, std::string &);
+template bool TileTuner::choose_saturated<double>(const Geometry &, int, hipStream_t, TileChoice &, std::string &);
 template bool TileTuner::choose<float>(const Geometry &, int, float *, hipStream_t, TileChoice &, std::string &);
 template bool TileTuner::choose<double>(const Geometry &, int, double *, hipStream_t, TileChoice &, std::string &);
 

@@ -91,6 +91,13 @@ public:
     // d_scratch: device room for g.points() values of Real (overwritten)
     template <class Real>
     bool choose(const Geometry &g, int kernel, Real *d_scratch, hipStream_t stream, TileChoice &out, std::string &err);
+    // Tile of a grid that runs inside a full launch (an ensemble member):
+    // the candidate with the least padded work per unit of its saturated
+    // throughput, the throughputs timed once per kernel and output type on
+    // a synthetic grid that fills the device with no padding for any
+    // candidate.
+    template <class Real>
+    bool choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err);
     int tuned_shapes() const { return static_cast<int>(cache_.size()); }
 
 private:
@@ -103,6 +110,10 @@ private:
     void *d_tab_ = nullptr, *d_part_ = nullptr;
     size_t tab_cap_ = 0, part_cap_ = 0;
     hipEvent_t ev_[2] = {nullptr, nullptr};
+    // saturated throughput (point-harmonics per ms) per (kernel, sizeof(Real), nbx, tk)
+    std::map<std::array<long, 4>, double> rate_;
+    void *d_sat_ = nullptr;  // output of the saturating grid
+    size_t sat_cap_ = 0;
 };
 
 // Monte Carlo ensembles: operand tables of every member, one persistent
