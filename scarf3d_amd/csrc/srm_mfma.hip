@@ -10,9 +10,10 @@
 //     of its own column and 8 K values and splits it into 16-bit parts
 //     directly in MFMA operand layout (row c = lane % 32,
 //     K = 8 (lane / 32) .. +7);
-//   - each wave reads all x chunks of a k half, then issues the products in
+//   - each wave reads all x chunks of a k half and issues the products in
 //     product-major order, so that the NBX accumulators' dependent chains
-//     interleave on the matrix core;
+//     interleave on the matrix core; the second half's LDS reads and operand
+//     formation are scheduled between the first half's products;
 //   - one vmcnt wait and one barrier end each stage.
 // Accumulation: FP32 MFMA accumulators, promoted into a second FP32
 // accumulator every 128 harmonics (8 stages), so the rounding of the long
@@ -145,6 +146,24 @@ __device__ inline void split2(float re, float im, uint32_t (&out)[SchemeOf<S>::p
 
 __device__ inline f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 __device__ inline f32x16 mma(f16x8 a, f16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+
+// Instruction-scheduler pattern for the block in front of it: NMF MFMAs, each
+// preceded by its share of NDS LDS reads and followed by its share of NVA
+// VALU instructions, spread evenly. Work that is independent of the MFMAs
+// (the next operand's formation) then issues between them, while each MFMA
+// keeps the matrix core busy for 32 cycles.
+template <int NMF, int NDS, int NVA, int I = 0>
+__device__ __forceinline__ void interleave_mfma()
+{
+    if constexpr (I < NMF) {
+        constexpr int ds = (I + 1) * NDS / NMF - I * NDS / NMF;
+        constexpr int va = (I + 1) * NVA / NMF - I * NVA / NMF;
+        if constexpr (ds > 0) __builtin_amdgcn_sched_group_barrier(0x100, ds, 0);  // LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                         // one MFMA
+        if constexpr (va > 0) __builtin_amdgcn_sched_group_barrier(0x002, va, 0);  // VALU
+        interleave_mfma<NMF, NDS, NVA, I + 1>();
+    }
+}
 
 // LDS position of 16-byte chunk q (K = 8q..8q+7) of row r (64 bytes per
 // row and part): the chunk index is XOR-permuted by row bits so that the
@@ -339,60 +358,89 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
     const int crow = wave * 32 + (lane & 31);  // generated-operand row (grid column) of this lane
     const int cj = crow & tj_mask, ck = crow >> tj_shift;
     const int khalf = lane >> 5;
-    // stage s from `cur` while stage s+1 streams into `nxt`
-    auto stage = [&](int s, StageLds<P> &cur, StageLds<P> &nxt) __attribute__((always_inline)) {
-        if (s + 1 < stages) issue_loads(s + 1, nxt);
+    // generated operand of k half kk: Z*Y for 4 harmonics x (re, im) of this
+    // lane's column (read_yz), split into P parts in MFMA operand layout (form_g)
+    auto read_yz = [&](const StageLds<P> &buf, int kk, float2 (&y)[4], float2 (&z)[4]) __attribute__((always_inline)) {
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            // generated operand: Z*Y for 4 harmonics x (re, im) of this lane's column
-            uint32_t gw[P][4];
+        for (int t = 0; t < 4; ++t) {
+            const int h = kk * 8 + khalf * 4 + t;
+            y[t] = buf.y[h * tj + cj];
+            z[t] = buf.z[h * tk + ck];
+        }
+    };
+    auto form_g = [&](const float2 (&y)[4], const float2 (&z)[4], V (&g)[P], V &g0s) __attribute__((always_inline)) {
+        uint32_t gw[P][4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int h = kk * 8 + khalf * 4 + t;
-                const float2 y = cur.y[h * tj + cj];
-                const float2 z = cur.z[h * tk + ck];
-                const float re = z.x * y.x - z.y * y.y;
-                const float im = z.x * y.y + z.y * y.x;
-                uint32_t w[P];
-                split2<S>(re, im, w);
+        for (int t = 0; t < 4; ++t) {
+            const float re = z[t].x * y[t].x - z[t].y * y[t].y;
+            const float im = z[t].x * y[t].y + z[t].y * y[t].x;
+            uint32_t w[P];
+            split2<S>(re, im, w);
 #pragma unroll
-                for (int p = 0; p < P; ++p) gw[p][t] = w[p];
-            }
-            V g[P];
+            for (int p = 0; p < P; ++p) gw[p][t] = w[p];
+        }
 #pragma unroll
-            for (int p = 0; p < P; ++p) g[p] = __builtin_bit_cast(V, uint4{gw[p][0], gw[p][1], gw[p][2], gw[p][3]});
-            V g0s{};
-            if constexpr (S == kF16x2s) g0s = g[0] * static_cast<E>(kLoScale);  // exact: power of two, |G| < 8
-            // all x chunks of this k half first, then the products in
-            // product-major order: the NBX accumulators' chains interleave
-            // (each accumulator still sees its products in the same order)
-            V xv[NBX][P];
+        for (int p = 0; p < P; ++p) g[p] = __builtin_bit_cast(V, uint4{gw[p][0], gw[p][1], gw[p][2], gw[p][3]});
+        if constexpr (S == kF16x2s) g0s = g[0] * static_cast<E>(kLoScale);  // exact: power of two, |G| < 8
+    };
+    // all x chunks of k half kk
+    auto read_x = [&](const StageLds<P> &buf, int kk, V (&xv)[NBX][P]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int bx = 0; bx < NBX; ++bx) {
+            const int slot = chunk_slot(bx * 32 + (lane & 31), kk * 2 + khalf);
+#pragma unroll
+            for (int p = 0; p < P; ++p) xv[bx][p] = __builtin_bit_cast(V, buf.x[p][slot]);
+        }
+    };
+    // the products of one k half in product-major order: the NBX accumulators'
+    // chains interleave (each accumulator still sees its products in order)
+    constexpr int NP = S == kBf16x3 ? 6 : 3;
+    auto products = [&](const V (&g)[P], const V &g0s, const V (&xv)[NBX][P]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
 #pragma unroll
             for (int bx = 0; bx < NBX; ++bx) {
-                const int slot = chunk_slot(bx * 32 + (lane & 31), kk * 2 + khalf);
-#pragma unroll
-                for (int p = 0; p < P; ++p) xv[bx][p] = __builtin_bit_cast(V, cur.x[p][slot]);
-            }
-            constexpr int NP = S == kBf16x3 ? 6 : 3;
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-#pragma unroll
-                for (int bx = 0; bx < NBX; ++bx) {
-                    f32x16 &a = acc[0][bx];
-                    if constexpr (S == kBf16x3) {
-                        constexpr int gi[6] = {1, 0, 2, 0, 1, 0}, xi[6] = {1, 2, 0, 1, 0, 0};
-                        a = mma(g[gi[q]], xv[bx][xi[q]], a);
-                    } else if constexpr (S == kBf16x2) {
-                        constexpr int gi[3] = {0, 1, 0}, xi[3] = {1, 0, 0};
-                        a = mma(g[gi[q]], xv[bx][xi[q]], a);
-                    } else {
-                        if (q == 0) a = mma(g[0], xv[bx][1], a);
-                        else if (q == 1) a = mma(g[1], xv[bx][0], a);
-                        else a = mma(g0s, xv[bx][0], a);
-                    }
+                f32x16 &a = acc[0][bx];
+                if constexpr (S == kBf16x3) {
+                    constexpr int gi[6] = {1, 0, 2, 0, 1, 0}, xi[6] = {1, 2, 0, 1, 0, 0};
+                    a = mma(g[gi[q]], xv[bx][xi[q]], a);
+                } else if constexpr (S == kBf16x2) {
+                    constexpr int gi[3] = {0, 1, 0}, xi[3] = {1, 0, 0};
+                    a = mma(g[gi[q]], xv[bx][xi[q]], a);
+                } else {
+                    if (q == 0) a = mma(g[0], xv[bx][1], a);
+                    else if (q == 1) a = mma(g[1], xv[bx][0], a);
+                    else a = mma(g0s, xv[bx][0], a);
                 }
             }
         }
+    };
+    // VALU instructions of one operand formation (4 harmonics: complex product
+    // and split), an estimate for the interleave pattern
+    constexpr int kFormValu = S == kBf16x3 ? 60 : 48;
+
+    // stage s from `cur` while stage s+1 streams into `nxt`. Three phases:
+    //   1. first-half operand G0 and its x chunks;
+    //   2. first-half products, with the second half's LDS reads and operand
+    //      formation issued between them (their own work for the matrix core
+    //      while the VALU forms G1);
+    //   3. second-half products.
+    auto stage = [&](int s, StageLds<P> &cur, StageLds<P> &nxt) __attribute__((always_inline)) {
+        if (s + 1 < stages) issue_loads(s + 1, nxt);
+        V g0[P], g0s0{}, g1[P], g0s1{};
+        V xv0[NBX][P], xv1[NBX][P];
+        float2 y0[4], z0[4], y1[4], z1[4];
+        read_yz(cur, 0, y0, z0);
+        read_yz(cur, 1, y1, z1);
+        form_g(y0, z0, g0, g0s0);
+        read_x(cur, 0, xv0);
+        __builtin_amdgcn_sched_barrier(0);
+        form_g(y1, z1, g1, g0s1);
+        products(g0, g0s0, xv0);
+        read_x(cur, 1, xv1);
+        interleave_mfma<NP * NBX, NBX * P, kFormValu>();
+        __builtin_amdgcn_sched_barrier(0);
+        products(g1, g0s1, xv1);
         if ((s % kPromote) == kPromote - 1 || s + 1 == stages) {
 #pragma unroll
             for (int bx = 0; bx < NBX; ++bx) {
@@ -1071,8 +1119,9 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
             }
         }
     }
-    // keep the least-padding tile unless another is clearly faster (timing noise)
-    if (def_ms > 0.0 && best_ms > 0.97 * def_ms) {
+    // keep the least-padding tile unless another is clearly faster: the
+    // candidates' short timings differ by a few percent from run to run
+    if (def_ms > 0.0 && best_ms > 0.95 * def_ms) {
         best.nbx = def.nbx;
         best.tk = def.tk;
         best_ms = def_ms;
