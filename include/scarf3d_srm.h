@@ -87,8 +87,9 @@ double scarf_ensemble_last_ms(void);
  * (j x k = 256 grid columns). The tile is picked per grid shape by a tuner
  * that times the candidates on a short synthetic harmonic record (SCARF_TUNE=0:
  * least padding instead; SCARF_TUNE_CACHE=<file> keeps the choices across
- * processes); it changes the speed only, never the field. Returns 3, or 0
- * before any MFMA run. */
+ * processes; SCARF_TILE=x,j,k fixes the tile of single-field runs, e.g.
+ * SCARF_TILE=128,32,8); it changes the speed only, never the field. Returns
+ * 3, or 0 before any MFMA run. */
 int scarf_last_tile(int out[3]);
 
 /* Device milliseconds of the last run: {tables, field, statistics,

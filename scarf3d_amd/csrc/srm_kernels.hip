@@ -569,7 +569,9 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
     const bool mfma = g.structured && (kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 ||
                                        kernel == kKernelMfmaBf16x2);
     TileChoice tile;
-    if (mfma && TileTuner::enabled() && !tuner().choose<Real>(g, kernel, d_field, stream_, tile, err)) return false;
+    if (mfma && !TileTuner::forced(tile) && TileTuner::enabled() &&
+        !tuner().choose<Real>(g, kernel, d_field, stream_, tile, err))
+        return false;
     timings_ = Timings{};
     HIP_TRY(hipEventRecord(ev_[0], stream_));
 

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -182,6 +183,14 @@ __device__ inline int chunk_slot(int r, int q) { return r * 4 + (q ^ chunk_perm(
 //   Y: [np][ny] float2 = (cos, sin)(2 pi ty)
 //   Z: [np][nz] float2 = amp * (cos, sin)(2 pi tz)
 // Phases are formed and reduced in FP64; sin/cos in FP64 (sincospi).
+// Work items: one x-point of one stage (its 16 harmonics, so that each part's
+// 64-byte row is written whole: consecutive threads write consecutive rows),
+// then one (harmonic, j) of Y, then one (harmonic, k) of Z.
+template <int S>
+__device__ inline uint32_t half_bits(typename SchemeOf<S>::elt v)
+{
+    return __builtin_bit_cast(uint16_t, v);
+}
 template <int S>
 __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restrict__ harm, int np_, int nx,
                                                            int nx_pad, int ny, int nz, int ox, int oy, int oz,
@@ -191,40 +200,46 @@ __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restr
 {
     using E = typename SchemeOf<S>::elt;
     constexpr int P = SchemeOf<S>::parts;
-    const int width = nx_pad + ny + nz;
-    const long total = long(np_) * width;
+    const long nxi = long(np_ / kStageH) * nx_pad, nyi = long(np_) * ny;
+    const long total = nxi + nyi + long(np_) * nz;
     for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < total; e += long(gridDim.x) * blockDim.x) {
-        const int n = int(e / width);
-        const int r = int(e % width);
-        if (r < nx_pad) {
-            float c = 0.f, s = 0.f;
-            if (r < nx) {
-                double t = (harm[n] * (double(r + ox) * ds)) * kInvTwoPi;
-                t -= rint(t);
-                double sd, cd;
-                sincospi(2.0 * t, &sd, &cd);
-                c = static_cast<float>(cd);
-                s = static_cast<float>(-sd);
+        if (e < nxi) {
+            const int stage = int(e / nx_pad), r = int(e % nx_pad);
+            uint32_t words[P][kStageH];  // harmonic h: (cos, -sin) parts as one little-endian word
+#pragma unroll
+            for (int h = 0; h < kStageH; ++h) {
+                const int n = stage * kStageH + h;
+                float c = 0.f, s = 0.f;
+                if (r < nx) {
+                    double t = (harm[n] * (double(r + ox) * ds)) * kInvTwoPi;
+                    t -= rint(t);
+                    double sd, cd;
+                    sincospi(2.0 * t, &sd, &cd);
+                    c = static_cast<float>(cd);
+                    s = static_cast<float>(-sd);
+                }
+                E pc[P], ps[P];
+                split<S>(c, pc);
+                split<S>(s, ps);
+#pragma unroll
+                for (int p = 0; p < P; ++p) words[p][h] = half_bits<S>(pc[p]) | (half_bits<S>(ps[p]) << 16);
             }
-            E pc[P], ps[P];
-            split<S>(c, pc);
-            split<S>(s, ps);
-            const int stage = n / kStageH, h = n % kStageH;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                E *row = xt + ((size_t(stage) * P + p) * nx_pad + r) * kStageK;
-                row[2 * h] = pc[p];
-                row[2 * h + 1] = ps[p];
+                uint4 *row = reinterpret_cast<uint4 *>(xt + ((size_t(stage) * P + p) * nx_pad + r) * kStageK);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    row[q] = uint4{words[p][4 * q], words[p][4 * q + 1], words[p][4 * q + 2], words[p][4 * q + 3]};
             }
-        } else if (r < nx_pad + ny) {
-            const int j = r - nx_pad;
+        } else if (e < nxi + nyi) {
+            const int n = int((e - nxi) / ny), j = int((e - nxi) % ny);
             double t = (harm[np_ + n] * (double(j + oy) * ds)) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
             yt[size_t(n) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
         } else {
-            const int k = r - nx_pad - ny;
+            const int n = int((e - nxi - nyi) / nz), k = int((e - nxi - nyi) % nz);
             double t = (harm[2 * np_ + n] * (double(k + oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
@@ -707,7 +722,7 @@ bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, 
     char *base = static_cast<char *>(d_tab);
     float2 *yt = reinterpret_cast<float2 *>(base + plan.x_bytes);
     float2 *zt = reinterpret_cast<float2 *>(base + plan.x_bytes + plan.y_bytes);
-    const long total = long(np_) * (plan.nx_pad + g.ny + g.nz);
+    const long total = long(np_ / kStageH) * plan.nx_pad + long(np_) * (g.ny + g.nz);  // work items
     const int blocks = int(std::min<long>((total + 255) / 256, 65536));
     switch (scheme_of(plan.kernel)) {
     case kBf16x3:
@@ -988,7 +1003,8 @@ template bool Ensemble::run<double>(hipStream_t, int, const std::vector<Ensemble
 // Tile autotuner
 namespace {
 
-constexpr int kTuneStages = 24;  // stages timed per candidate (384 harmonics)
+constexpr int kTuneStages = 64;     // stages timed per single-field candidate (1024 harmonics)
+constexpr int kTuneStagesSat = 24;  // stages per candidate on the device-filling rate-table grid
 
 // synthetic record: distinct small wavenumbers, unit amplitudes
 __global__ void tune_record(double *rec, int np_)
@@ -1023,10 +1039,25 @@ TileTuner::~TileTuner()
 {
     if (d_sat_) (void)hipFree(d_sat_);
     if (d_rec_) (void)hipFree(d_rec_);
+    if (d_rec_sat_) (void)hipFree(d_rec_sat_);
     if (d_tab_) (void)hipFree(d_tab_);
     if (d_part_) (void)hipFree(d_part_);
     for (hipEvent_t &e : ev_)
         if (e) (void)hipEventDestroy(e);
+}
+
+bool TileTuner::prepare(double **rec, int np_, hipStream_t stream, std::string &err)
+{
+    if (!ev_[0]) {
+        ENS_TRY(hipEventCreate(&ev_[0]));
+        ENS_TRY(hipEventCreate(&ev_[1]));
+    }
+    if (!*rec) {
+        ENS_TRY(hipMalloc(rec, size_t(5) * np_ * sizeof(double)));
+        tune_record<<<(np_ + 255) / 256, 256, 0, stream>>>(*rec, np_);
+        ENS_TRY(hipGetLastError());
+    }
+    return true;
 }
 
 bool TileTuner::enabled()
@@ -1066,6 +1097,21 @@ void TileTuner::save_disk(const Key &k, const TileChoice &c)
     std::fclose(f);
 }
 
+// SCARF_TILE=x,j,k (x in {64, 96, 128}, j * k = 256, k in {1, 2, 4, 8})
+// fixes the block tile of single-field runs, e.g. to profile one tile
+bool TileTuner::forced(TileChoice &out)
+{
+    const char *e = std::getenv("SCARF_TILE");
+    if (!e || !*e) return false;
+    int x = 0, j = 0, k = 0;
+    if (std::sscanf(e, "%d,%d,%d", &x, &j, &k) != 3) return false;
+    if ((x != 64 && x != 96 && x != 128) || (k != 1 && k != 2 && k != 4 && k != 8) || j * k != kTileC) return false;
+    out = TileChoice{};
+    out.nbx = x / 32;
+    out.tk = k;
+    return true;
+}
+
 template <class Real>
 bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream_t stream, TileChoice &out,
                        std::string &err)
@@ -1080,13 +1126,7 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
     const int np_ = kTuneStages * kStageH;
     MfmaPlan def;
     if (!mfma_plan(g, np_, kernel, def, err)) return false;
-    if (!d_rec_) {
-        ENS_TRY(hipMalloc(&d_rec_, size_t(5) * np_ * sizeof(double)));
-        ENS_TRY(hipEventCreate(&ev_[0]));
-        ENS_TRY(hipEventCreate(&ev_[1]));
-        tune_record<<<(np_ + 255) / 256, 256, 0, stream>>>(d_rec_, np_);
-        ENS_TRY(hipGetLastError());
-    }
+    if (!prepare(&d_rec_, np_, stream, err)) return false;
     auto padded = [&](const MfmaPlan &p) { return double(p.blocks) * kTileC * 32.0 * p.nbx; };
     TileChoice best;
     double best_ms = -1.0, def_ms = -1.0;
@@ -1138,7 +1178,7 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
 template <class Real>
 bool TileTuner::choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err)
 {
-    const int np_ = kTuneStages * kStageH;
+    const int np_ = kTuneStagesSat * kStageH;
     const long ks = long(sizeof(Real));
     // the throughput table of this kernel and output type, timed once
     if (rate_.find({long(kernel), ks, 4L, 8L}) == rate_.end()) {
@@ -1149,13 +1189,7 @@ bool TileTuner::choose_saturated(const Geometry &g, int kernel, hipStream_t stre
         sg.nz = 128;
         sg.ds = 10.0;
         if (!tune_reserve(&d_sat_, sat_cap_, size_t(sg.points()) * sizeof(Real), err)) return false;
-        if (!d_rec_) {
-            ENS_TRY(hipMalloc(&d_rec_, size_t(5) * np_ * sizeof(double)));
-            ENS_TRY(hipEventCreate(&ev_[0]));
-            ENS_TRY(hipEventCreate(&ev_[1]));
-            tune_record<<<(np_ + 255) / 256, 256, 0, stream>>>(d_rec_, np_);
-            ENS_TRY(hipGetLastError());
-        }
+        if (!prepare(&d_rec_sat_, np_, stream, err)) return false;
         Real *scratch = static_cast<Real *>(d_sat_);
         for (int nbx = 4; nbx >= 2; --nbx) {
             for (int tk = 8; tk >= 1; tk /= 2) {
@@ -1164,7 +1198,7 @@ bool TileTuner::choose_saturated(const Geometry &g, int kernel, hipStream_t stre
                 if (p.nbx != nbx || p.tk != tk) continue;
                 if (!tune_reserve(&d_tab_, tab_cap_, p.table_bytes, err)) return false;
                 if (!tune_reserve(&d_part_, part_cap_, size_t(3) * p.nparts * sizeof(double), err)) return false;
-                if (!mfma_tables(p, sg, d_rec_, np_, d_tab_, stream, err)) return false;
+                if (!mfma_tables(p, sg, d_rec_sat_, np_, d_tab_, stream, err)) return false;
                 if (!mfma_field(p, sg, d_tab_, 1.0, scratch, static_cast<double *>(d_part_), stream, err)) return false;
                 float t_min = -1.f;
                 for (int rep = 0; rep < 3; ++rep) {

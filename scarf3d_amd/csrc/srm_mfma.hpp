@@ -98,15 +98,21 @@ public:
     // candidate.
     template <class Real>
     bool choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err);
+    static bool forced(TileChoice &out);  // SCARF_TILE, when set to a valid tile
     int tuned_shapes() const { return static_cast<int>(cache_.size()); }
 
 private:
     using Key = std::array<long, 5>;
     void load_disk();
     void save_disk(const Key &k, const TileChoice &c);
+    // the synthetic record of np_ harmonics in *rec (made on first use; its
+    // SoA layout depends on np_, so each timing length has its own) and the
+    // timing events
+    bool prepare(double **rec, int np_, hipStream_t stream, std::string &err);
     std::map<Key, TileChoice> cache_;
     bool disk_loaded_ = false;
-    double *d_rec_ = nullptr;
+    double *d_rec_ = nullptr;      // single-field candidates
+    double *d_rec_sat_ = nullptr;  // saturated-rate table
     void *d_tab_ = nullptr, *d_part_ = nullptr;
     size_t tab_cap_ = 0, part_cap_ = 0;
     hipEvent_t ev_[2] = {nullptr, nullptr};

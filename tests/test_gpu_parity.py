@@ -192,3 +192,28 @@ def test_tile_tuner_changes_speed_not_field(oracle, lib_f, monkeypatch, tmp_path
     print(f"  tuned tile {tile}, least-padding tile {lib_f.last_tile()}, "
           f"bit-identical: {np.array_equal(tuned, plain)}", flush=True)
     assert np.abs(tuned.astype(np.float64) - plain).max() <= 1e-7 * cfg["sigma"]
+
+
+@pytest.mark.parametrize("kernel", ["mfma_bf16x3", "mfma_f16x2"])
+def test_every_forced_tile_gives_the_same_field(lib_f, monkeypatch, kernel):
+    """SCARF_TILE fixes the block tile: every x width (64/96/128) and column
+    shape (256x1 .. 32x8) runs, reports itself through scarf_last_tile, and
+    sums the same products per output value as the least-padding tile (ragged
+    in x, j and k)."""
+    cfg = structured_cfg(n=(150, 70, 11), nharm=320, real=np.float32)
+    monkeypatch.setenv("SCARF_TUNE", "0")
+    _say("least-padding tile", kernel, cfg["fe"])
+    ref, _ = _run(lib_f, cfg, 5, kernel)
+    for x in (64, 96, 128):
+        for k in (1, 2, 4, 8):
+            tile = f"{x},{256 // k},{k}"
+            monkeypatch.setenv("SCARF_TILE", tile)
+            _say("forced tile", kernel, tile)
+            lib_f.finalize()
+            field, _ = _run(lib_f, cfg, 5, kernel)
+            assert lib_f.last_tile() == (x, 256 // k, k)
+            assert np.abs(field.astype(np.float64) - ref).max() <= 1e-7 * cfg["sigma"], tile
+    monkeypatch.setenv("SCARF_TILE", "100,2,128")  # not a tile: ignored
+    lib_f.finalize()
+    field, _ = _run(lib_f, cfg, 5, kernel)
+    assert np.abs(field.astype(np.float64) - ref).max() <= 1e-7 * cfg["sigma"]
