@@ -191,18 +191,28 @@ __device__ inline uint32_t half_bits(typename SchemeOf<S>::elt v)
 {
     return __builtin_bit_cast(uint16_t, v);
 }
+// one realization's table parameters (nx_pad, and the table pointers, from its plan)
+struct TableArgs {
+    const double *harm;
+    void *xt;
+    float2 *yt, *zt;
+    int np_, nx, nx_pad, ny, nz, ox, oy, oz;
+    double ds, zscale;
+};
+
+// the work items first, first + step, ... of one realization's tables
 template <int S>
-__global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restrict__ harm, int np_, int nx,
-                                                           int nx_pad, int ny, int nz, int ox, int oy, int oz,
-                                                           double ds, double zscale,
-                                                           typename SchemeOf<S>::elt *__restrict__ xt,
-                                                           float2 *__restrict__ yt, float2 *__restrict__ zt)
+__device__ __forceinline__ void table_items(const TableArgs &a, long first, long step)
 {
     using E = typename SchemeOf<S>::elt;
     constexpr int P = SchemeOf<S>::parts;
+    const double *__restrict__ harm = a.harm;
+    E *__restrict__ xt = static_cast<E *>(a.xt);
+    const int np_ = a.np_, nx = a.nx, nx_pad = a.nx_pad, ny = a.ny, nz = a.nz;
+    const double ds = a.ds;
     const long nxi = long(np_ / kStageH) * nx_pad, nyi = long(np_) * ny;
     const long total = nxi + nyi + long(np_) * nz;
-    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < total; e += long(gridDim.x) * blockDim.x) {
+    for (long e = first; e < total; e += step) {
         if (e < nxi) {
             const int stage = int(e / nx_pad), r = int(e % nx_pad);
             uint32_t words[P][kStageH];  // harmonic h: (cos, -sin) parts as one little-endian word
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restr
                 const int n = stage * kStageH + h;
                 float c = 0.f, s = 0.f;
                 if (r < nx) {
-                    double t = (harm[n] * (double(r + ox) * ds)) * kInvTwoPi;
+                    double t = (harm[n] * (double(r + a.ox) * ds)) * kInvTwoPi;
                     t -= rint(t);
                     double sd, cd;
                     sincospi(2.0 * t, &sd, &cd);
@@ -233,21 +243,36 @@ __global__ __launch_bounds__(256) void mfma_operand_tables(const double *__restr
             }
         } else if (e < nxi + nyi) {
             const int n = int((e - nxi) / ny), j = int((e - nxi) % ny);
-            double t = (harm[np_ + n] * (double(j + oy) * ds)) * kInvTwoPi;
+            double t = (harm[np_ + n] * (double(j + a.oy) * ds)) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
-            yt[size_t(n) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
+            a.yt[size_t(n) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
         } else {
             const int n = int((e - nxi - nyi) / nz), k = int((e - nxi - nyi) % nz);
-            double t = (harm[2 * np_ + n] * (double(k + oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
+            double t = (harm[2 * np_ + n] * (double(k + a.oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
-            const double a = harm[4 * np_ + n] * zscale;
-            zt[size_t(n) * nz + k] = make_float2(static_cast<float>(a * cd), static_cast<float>(a * sd));
+            const double amp = harm[4 * np_ + n] * a.zscale;
+            a.zt[size_t(n) * nz + k] = make_float2(static_cast<float>(amp * cd), static_cast<float>(amp * sd));
         }
     }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void mfma_operand_tables(TableArgs a)
+{
+    table_items<S>(a, blockIdx.x * long(blockDim.x) + threadIdx.x, long(gridDim.x) * blockDim.x);
+}
+
+// the tables of every member of an ensemble in one launch: blockIdx.y is the
+// member, the blocks along x share its work items
+template <int S>
+__global__ __launch_bounds__(256) void ensemble_tables(const TableArgs *__restrict__ args)
+{
+    const TableArgs a = args[blockIdx.y];
+    table_items<S>(a, blockIdx.x * long(blockDim.x) + threadIdx.x, long(gridDim.x) * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -716,32 +741,41 @@ double mfma_zscale(int kernel, double amp_max)
     return z;
 }
 
+namespace {
+
+TableArgs table_args(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab)
+{
+    char *base = static_cast<char *>(d_tab);
+    TableArgs a;
+    a.harm = d_harm;
+    a.xt = base;
+    a.yt = reinterpret_cast<float2 *>(base + plan.x_bytes);
+    a.zt = reinterpret_cast<float2 *>(base + plan.x_bytes + plan.y_bytes);
+    a.np_ = np_;
+    a.nx = g.nx;
+    a.nx_pad = plan.nx_pad;
+    a.ny = g.ny;
+    a.nz = g.nz;
+    a.ox = g.ox;
+    a.oy = g.oy;
+    a.oz = g.oz;
+    a.ds = g.ds;
+    a.zscale = plan.zscale;
+    return a;
+}
+
+}  // namespace
+
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err)
 {
-    char *base = static_cast<char *>(d_tab);
-    float2 *yt = reinterpret_cast<float2 *>(base + plan.x_bytes);
-    float2 *zt = reinterpret_cast<float2 *>(base + plan.x_bytes + plan.y_bytes);
+    const TableArgs a = table_args(plan, g, d_harm, np_, d_tab);
     const long total = long(np_ / kStageH) * plan.nx_pad + long(np_) * (g.ny + g.nz);  // work items
     const int blocks = int(std::min<long>((total + 255) / 256, 65536));
     switch (scheme_of(plan.kernel)) {
-    case kBf16x3:
-        mfma_operand_tables<kBf16x3><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds, plan.zscale,
-                                                                  reinterpret_cast<__bf16 *>(base),
-                                                                  yt, zt);
-        break;
-    case kBf16x2:
-        mfma_operand_tables<kBf16x2><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds, plan.zscale,
-                                                                  reinterpret_cast<__bf16 *>(base),
-                                                                  yt, zt);
-        break;
-    default:
-        mfma_operand_tables<kF16x2s><<<blocks, 256, 0, stream>>>(d_harm, np_, g.nx, plan.nx_pad, g.ny, g.nz, g.ox,
-                                                                  g.oy, g.oz, g.ds, plan.zscale,
-                                                                  reinterpret_cast<_Float16 *>(base), yt, zt);
-        break;
+    case kBf16x3: mfma_operand_tables<kBf16x3><<<blocks, 256, 0, stream>>>(a); break;
+    case kBf16x2: mfma_operand_tables<kBf16x2><<<blocks, 256, 0, stream>>>(a); break;
+    default: mfma_operand_tables<kF16x2s><<<blocks, 256, 0, stream>>>(a); break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -805,10 +839,10 @@ void Ensemble::release()
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
     graph_ = nullptr;
-    void *bufs[] = {d_rec_, d_tab_, d_part_, d_desc_, d_items_, d_stat_};
+    void *bufs[] = {d_rec_, d_tab_, d_part_, d_desc_, d_items_, d_stat_, d_targs_};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    d_rec_ = d_tab_ = d_part_ = d_desc_ = d_items_ = d_stat_ = nullptr;
+    d_rec_ = d_tab_ = d_part_ = d_desc_ = d_items_ = d_stat_ = d_targs_ = nullptr;
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     ev_[0] = ev_[1] = nullptr;
@@ -834,8 +868,8 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         return false;
     }
     const int n = static_cast<int>(members.size());
-    if (n < 1) {
-        err = "empty ensemble";
+    if (n < 1 || n > 65535) {
+        err = n < 1 ? "empty ensemble" : "more than 65535 ensemble members";
         return false;
     }
     // plans, offsets and the shape key
@@ -935,16 +969,21 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         const int persistent = std::max(1, std::min(nitems_, cus));
 
         // capture the launch sequence
+        // operand-table parameters of every member (one launch builds all tables)
+        std::vector<TableArgs> targs(n);
+        for (int m = 0; m < n; ++m)
+            targs[m] = table_args(plans[m], members[m].g, static_cast<const double *>(d_rec_) + rec_off[m],
+                                  members[m].np_, static_cast<char *>(d_tab_) + tab_off[m]);
+        ENS_TRY(hipMalloc(&d_targs_, size_t(n) * sizeof(TableArgs)));
+        ENS_TRY(hipMemcpy(d_targs_, targs.data(), size_t(n) * sizeof(TableArgs), hipMemcpyHostToDevice));
+
         ENS_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int m = 0; m < n; ++m) {
-            const double *rec = static_cast<const double *>(d_rec_) + rec_off[m];
-            if (!mfma_tables(plans[m], members[m].g, rec, members[m].np_, static_cast<char *>(d_tab_) + tab_off[m],
-                             stream, err)) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(stream, &g);
-                if (g) (void)hipGraphDestroy(g);
-                return false;
-            }
+        const TableArgs *dt = static_cast<const TableArgs *>(d_targs_);
+        const dim3 tgrid(256, static_cast<unsigned>(n));  // 256 blocks share each member's work items
+        switch (s) {
+        case kBf16x3: ensemble_tables<kBf16x3><<<tgrid, 256, 0, stream>>>(dt); break;
+        case kBf16x2: ensemble_tables<kBf16x2><<<tgrid, 256, 0, stream>>>(dt); break;
+        default: ensemble_tables<kF16x2s><<<tgrid, 256, 0, stream>>>(dt); break;
         }
         const MemberDesc *dd = static_cast<const MemberDesc *>(d_desc_);
         const int2 *di = static_cast<const int2 *>(d_items_);

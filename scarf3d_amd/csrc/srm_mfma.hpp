@@ -122,7 +122,7 @@ private:
     size_t sat_cap_ = 0;
 };
 
-// Monte Carlo ensembles: operand tables of every member, one persistent
+// Monte Carlo ensembles: operand tables of every member (one launch), one persistent
 // grouped-GEMM launch over all (member, tile) work items (sorted by cost,
 // largest first, and dealt to the blocks round-robin), per-member
 // statistics and rescale. The launch sequence is captured once per ensemble shape into a
@@ -151,6 +151,7 @@ private:
     hipEvent_t ev_[2] = {nullptr, nullptr};
     void *d_rec_ = nullptr, *d_tab_ = nullptr, *d_part_ = nullptr, *d_desc_ = nullptr, *d_items_ = nullptr;
     void *d_stat_ = nullptr;
+    void *d_targs_ = nullptr;  // per-member operand-table parameters
     size_t rec_bytes_ = 0;
     int nitems_ = 0, nmembers_ = 0;
     std::vector<unsigned char> desc_host_;
