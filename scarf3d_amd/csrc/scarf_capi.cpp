@@ -426,7 +426,10 @@ void scarf_io_one(const int nd, const int *npts, const real *field, const char f
         return;
     }
     size_t n = 1;
-    for (int i = 0; i < nd; ++i) n *= size_t(npts[i]);
+    for (int i = 0; i < nd; ++i) {
+        if (npts[i] < 1) return (void)fail("scarf_io_one: npts must be positive");
+        n *= size_t(npts[i]);
+    }
     FILE *f = std::fopen(fname, "wb");
     if (!f || std::fwrite(field, sizeof(real), n, f) != n) fail(std::string("scarf_io_one: cannot write ") + fname);
     if (f) std::fclose(f);
@@ -439,6 +442,7 @@ void scarf_io_slice(const int *npts, const char axis[], const int plane, const r
         return;
     }
     const long nx = npts[0], ny = npts[1], nz = npts[2];
+    if (nx < 1 || ny < 1 || nz < 1) return (void)fail("scarf_io_slice: npts must be positive");
     std::vector<real> out;
     const long p = plane - 1;  // plane is 1-based
     if (axis[0] == 'x') {
@@ -684,6 +688,7 @@ int scarf_setup_values(double *out, int n)
 {
     State &s = state();
     if (!s.initialized) return fail("not initialized");
+    if (!out) return fail("scarf_setup_values: out is NULL");
     s.cfg.nharm = s.nharm;
     scarf::SrmSetup su;
     std::string err;
@@ -736,6 +741,7 @@ int scarf_get_config(double *out, int n)
 {
     State &s = state();
     if (!s.initialized) return fail("not initialized");
+    if (!out) return fail("scarf_get_config: out is NULL");
     const SrmConfig &c = s.cfg;
     const double v[32] = {double(c.nd), double(c.structured), double(c.fs[0]), double(c.fs[1]), double(c.fs[2]),
                           double(c.fe[0]), double(c.fe[1]), double(c.fe[2]), c.ds, c.dh, double(c.acf), c.cl[0],
@@ -751,6 +757,7 @@ long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
 {
     State &s = state();
     if (!s.initialized) return fail("not initialized");
+    if (!harm || !used) return fail("scarf_draw_harmonics: harm and used must be non-NULL");
     s.cfg.nharm = s.nharm;
     scarf::SrmSetup su;
     std::string err;

@@ -244,3 +244,12 @@ def test_io_writers(lib_f, tmp_path):
         lib_f.io_slice(n, axis, plane, f, tmp_path / f"s{axis}.bin")
         got = np.fromfile(tmp_path / f"s{axis}.bin", dtype=np.float32).reshape(ref.shape, order="F")
         assert np.array_equal(got, ref), axis
+    # bad sizes, planes and axes are reported and write nothing
+    lib_f.io_one((5, 0, 3), f, tmp_path / "bad.bin")
+    assert "npts must be positive" in lib_f.last_error() and not (tmp_path / "bad.bin").exists()
+    lib_f.io_slice((5, -4, 3), "z", 1, f, tmp_path / "bad.bin")
+    assert "npts must be positive" in lib_f.last_error() and not (tmp_path / "bad.bin").exists()
+    lib_f.io_slice(n, "x", 6, f, tmp_path / "bad.bin")
+    assert "plane out of range" in lib_f.last_error() and not (tmp_path / "bad.bin").exists()
+    lib_f.io_slice(n, "w", 1, f, tmp_path / "bad.bin")
+    assert "axis must be x, y or z" in lib_f.last_error() and not (tmp_path / "bad.bin").exists()
