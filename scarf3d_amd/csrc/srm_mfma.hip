@@ -409,19 +409,31 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
         }
     };
     auto form_g = [&](const float2 (&y)[4], const float2 (&z)[4], V (&g)[P], V &g0s) __attribute__((always_inline)) {
-        uint32_t gw[P][4];
+        uint32_t gw[P][4], gsw[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const float re = z[t].x * y[t].x - z[t].y * y[t].y;
             const float im = z[t].x * y[t].y + z[t].y * y[t].x;
-            uint32_t w[P];
-            split2<S>(re, im, w);
+            if constexpr (S == kF16x2s) {
+                // high part h, its 2^12 multiple (exact in fp16: |G| < 8), and the low
+                // part 2^12 (x - h) = fma(x, 2^12, -2^12 h): the difference is exact, so the
+                // fma rounds nothing and the low part equals the split2 one
+                const uint32_t h = pack2<S>(re, im);
+                const f16x2v hs = __builtin_bit_cast(f16x2v, h) * f16x2v{static_cast<E>(kLoScale), static_cast<E>(kLoScale)};
+                gw[0][t] = h;
+                gw[1][t] = pack2<S>(fmaf(re, kLoScale, -static_cast<float>(hs[0])),
+                                    fmaf(im, kLoScale, -static_cast<float>(hs[1])));
+                gsw[t] = __builtin_bit_cast(uint32_t, hs);
+            } else {
+                uint32_t w[P];
+                split2<S>(re, im, w);
 #pragma unroll
-            for (int p = 0; p < P; ++p) gw[p][t] = w[p];
+                for (int p = 0; p < P; ++p) gw[p][t] = w[p];
+            }
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) g[p] = __builtin_bit_cast(V, uint4{gw[p][0], gw[p][1], gw[p][2], gw[p][3]});
-        if constexpr (S == kF16x2s) g0s = g[0] * static_cast<E>(kLoScale);  // exact: power of two, |G| < 8
+        if constexpr (S == kF16x2s) g0s = __builtin_bit_cast(V, uint4{gsw[0], gsw[1], gsw[2], gsw[3]});
     };
     // all x chunks of k half kk
     auto read_x = [&](const StageLds<P> &buf, int kk, V (&xv)[NBX][P]) __attribute__((always_inline)) {
@@ -789,13 +801,13 @@ namespace {
 
 template <int S, class Real>
 void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, double scaling, Real *d_field,
-                 double *d_part, hipStream_t stream)
+                 double *d_part, hipStream_t stream, long blocks)
 {
     using E = typename SchemeOf<S>::elt;
     const E *xt = reinterpret_cast<const E *>(base);
     const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes);
     const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes);
-    const dim3 grid(static_cast<unsigned>(plan.blocks));
+    const dim3 grid(static_cast<unsigned>(blocks));
 #define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x, plan.tj_shift, plan.tk, \
                         plan.tiles_j, scaling, d_field, d_part
     switch (plan.nbx) {
@@ -810,14 +822,15 @@ void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, doub
 
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err)
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks)
 {
     const char *base = static_cast<const char *>(d_tab);
     scaling /= plan.zscale;  // exact: a power of two
+    const long blocks = max_blocks > 0 ? std::min(plan.blocks, max_blocks) : plan.blocks;
     switch (scheme_of(plan.kernel)) {
-    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
-    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
-    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream); break;
+    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
+    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
+    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -828,7 +841,7 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
 }
 
 template bool mfma_field<float>(const MfmaPlan &, const Geometry &, const void *, double, float *, double *,
-                                hipStream_t, std::string &);
+                                hipStream_t, std::string &, long);
 
 // ---------------------------------------------------------------------------
 Ensemble::~Ensemble() { release(); }
@@ -1042,7 +1055,11 @@ template bool Ensemble::run<double>(hipStream_t, int, const std::vector<Ensemble
 // Tile autotuner
 namespace {
 
-constexpr int kTuneStages = 64;     // stages timed per single-field candidate (1024 harmonics)
+// Single-field candidates run the real grid's first tiles, one block per CU
+// (the kernel's steady state: LDS holds one block per CU), for 320 stages
+// (5120 harmonics: an x table far larger than the caches, as in real runs).
+// Short runs on a small table ranked the tiles differently from full runs.
+constexpr int kTuneStages = 320;
 constexpr int kTuneStagesSat = 24;  // stages per candidate on the device-filling rate-table grid
 
 // synthetic record: distinct small wavenumbers, unit amplitudes
@@ -1166,6 +1183,9 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
     MfmaPlan def;
     if (!mfma_plan(g, np_, kernel, def, err)) return false;
     if (!prepare(&d_rec_, np_, stream, err)) return false;
+    int dev = 0, cus = 256;
+    ENS_TRY(hipGetDevice(&dev));
+    ENS_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     auto padded = [&](const MfmaPlan &p) { return double(p.blocks) * kTileC * 32.0 * p.nbx; };
     TileChoice best;
     double best_ms = -1.0, def_ms = -1.0;
@@ -1178,11 +1198,11 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
             if (!tune_reserve(&d_tab_, tab_cap_, p.table_bytes, err)) return false;
             if (!tune_reserve(&d_part_, part_cap_, size_t(3) * p.nparts * sizeof(double), err)) return false;
             if (!mfma_tables(p, g, d_rec_, np_, d_tab_, stream, err)) return false;
-            if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err)) return false;
+            if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err, cus)) return false;
             float t_min = -1.f;
             for (int rep = 0; rep < 3; ++rep) {
                 ENS_TRY(hipEventRecord(ev_[0], stream));
-                if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err))
+                if (!mfma_field(p, g, d_tab_, 1.0, d_scratch, static_cast<double *>(d_part_), stream, err, cus))
                     return false;
                 ENS_TRY(hipEventRecord(ev_[1], stream));
                 ENS_TRY(hipEventSynchronize(ev_[1]));
@@ -1190,9 +1210,11 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
                 ENS_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
                 if (t_min < 0.f || ms < t_min) t_min = ms;
             }
-            if (is_def) def_ms = t_min;
-            if (best_ms < 0.0 || t_min < best_ms) {
-                best_ms = t_min;
+            // the full grid runs in ceil(blocks / CUs) such waves
+            const double est = double(t_min) * double((p.blocks + cus - 1) / cus);
+            if (is_def) def_ms = est;
+            if (best_ms < 0.0 || est < best_ms) {
+                best_ms = est;
                 best.nbx = p.nbx;
                 best.tk = p.tk;
             }
@@ -1284,6 +1306,6 @@ template bool TileTuner::choose<float>(const Geometry &, int, float *, hipStream
 template bool TileTuner::choose<double>(const Geometry &, int, double *, hipStream_t, TileChoice &, std::string &);
 
 template bool mfma_field<double>(const MfmaPlan &, const Geometry &, const void *, double, double *, double *,
-                                 hipStream_t, std::string &);
+                                 hipStream_t, std::string &, long);
 
 }  // namespace scarf

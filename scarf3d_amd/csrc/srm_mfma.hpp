@@ -47,9 +47,11 @@ double mfma_zscale(int kernel, double amp_max);
 bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0, int tk = 0);
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err);
+// max_blocks > 0 launches only the first max_blocks tiles (the tile tuner's
+// timing runs); the field is then incomplete
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err);
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks = 0);
 
 // One realization of an ensemble, prepared on the host.
 struct EnsembleMember {
