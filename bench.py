@@ -320,7 +320,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SRM shards by z-slab ranges with no exchange inside the harmonic "
-              "loop, and multi-GPU runs are not part of this build; run with --gpus 1", file=sys.stderr)
+              "loop, and multi-GPU runs are not implemented; run with --gpus 1", file=sys.stderr)
         return 2
 
     if args.workload == "ensemble_64":
