@@ -612,7 +612,6 @@ int scarf_ensemble_execute(int n, const struct scarf_member *members, real *cons
             scarf::TileChoice tc;
             if (!s.device->tuner().choose_saturated<real>(em[m].g, kernel, s.device->stream(), tc, terr))
                 return fail("tile tuner: " + terr);
-            em[m].nbx = tc.nbx;
             em[m].tk = tc.tk;
         }
     }

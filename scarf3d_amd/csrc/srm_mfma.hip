@@ -558,7 +558,8 @@ struct MemberDesc {
     const void *xt;
     const float2 *yt, *zt;
     void *field;
-    int nx, ny, nz, nx_pad, stages, tiles_x, tiles_j, tiles_k, tj_shift, tk, nbx;
+    int nx, ny, nz, nx_pad, stages, tiles_x, tiles_j, tiles_k, tj_shift, tk;
+    int x_full, x_tail0, x_tail1;  // mixed x tiles: x_full of 128 points, then tails of 32 * x_tail points
     int part_base;  // first statistics triple of this member
     double scaling, sigma;
 };
@@ -634,14 +635,21 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_ensemble(const MemberDesc *_
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 item = items[it];  // (member, tile within the member, x fastest)
         const MemberDesc m = members[item.x];
-        const int x0 = (item.y % m.tiles_x) * 32 * m.nbx;
+        const int xi = item.y % m.tiles_x;  // x tile: full ones first, then the tails
+        int x0 = 128 * xi, nbx = 4;
+        if (xi == m.x_full) {
+            nbx = m.x_tail0;
+        } else if (xi > m.x_full) {
+            x0 = 128 * m.x_full + 32 * m.x_tail0;
+            nbx = m.x_tail1;
+        }
         const int ct = item.y / m.tiles_x;
         const int j0 = (ct % m.tiles_j) << m.tj_shift;
         const int k0 = (ct / m.tiles_j) * m.tk;
         const E *xt = static_cast<const E *>(m.xt);
         Out *field = static_cast<Out *>(m.field);
         double *po = part + 3 * (size_t(m.part_base) + item.y);
-        switch (m.nbx) {
+        switch (nbx) {
         case 2:
             gemm_tile<S, 2, Out>(xt, m.yt, m.zt, m.nx, m.ny, m.nz, m.nx_pad, m.stages, m.tj_shift, m.tk, x0, j0,
                                  k0, m.scaling, field, po, lds0, lds1);
@@ -735,6 +743,43 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
                                   : x_table_bytes<kBf16x2>(plan.nx_pad, np_);
     plan.y_bytes = size_t(np_) * g.ny * sizeof(float2);
     plan.z_bytes = size_t(np_) * g.nz * sizeof(float2);
+    plan.table_bytes = (plan.x_bytes + plan.y_bytes + plan.z_bytes + 255) / 256 * 256;
+    return true;
+}
+
+bool mfma_plan_mixed(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int tk)
+{
+    if (!mfma_plan(g, np_, kernel, plan, err, 4, tk)) return false;
+    const int chunks = (g.nx + 31) / 32;  // 32-point chunks of x
+    int full = chunks / 4, t0 = 0, t1 = 0;
+    switch (chunks % 4) {
+    case 1:  // 128 + 32 = 96 + 64; a single chunk takes one 64-point tile
+        if (full >= 1) {
+            --full;
+            t0 = 3;
+            t1 = 2;
+        } else {
+            t0 = 2;
+        }
+        break;
+    case 2: t0 = 2; break;
+    case 3: t0 = 3; break;
+    default: break;
+    }
+    plan.mixed = true;
+    plan.x_full = full;
+    plan.x_tail[0] = t0;
+    plan.x_tail[1] = t1;
+    plan.tiles_x = full + (t0 > 0) + (t1 > 0);
+    plan.nx_pad = 32 * (4 * full + t0 + t1);
+    plan.blocks = long(plan.tiles_x) * plan.tiles_j * plan.tiles_k;
+    if (plan.blocks > (1L << 30)) {
+        err = "grid too large for one launch";
+        return false;
+    }
+    plan.nparts = int(plan.blocks);
+    plan.x_bytes = (scheme_of(kernel) == kBf16x3) ? x_table_bytes<kBf16x3>(plan.nx_pad, np_)
+                                                  : x_table_bytes<kBf16x2>(plan.nx_pad, np_);
     plan.table_bytes = (plan.x_bytes + plan.y_bytes + plan.z_bytes + 255) / 256 * 256;
     return true;
 }
@@ -901,7 +946,7 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
             err = "ensemble members must be structured grids";
             return false;
         }
-        if (!mfma_plan(e.g, e.np_, kernel, plans[m], err, e.nbx, e.tk)) return false;
+        if (!mfma_plan_mixed(e.g, e.np_, kernel, plans[m], err, e.tk)) return false;
         plans[m].zscale = mfma_zscale(kernel, e.amp_max);
         rec_off[m] = rec_total;
         rec_total += size_t(5) * e.np_;
@@ -924,15 +969,22 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
         mix(bits);
     }
 
-    // work items, longest members first (all tiles of a member cost the same)
-    std::vector<int> order(n);
-    for (int m = 0; m < n; ++m) order[m] = m;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return long(plans[a].stages) * plans[a].nbx > long(plans[b].stages) * plans[b].nbx;
-    });
+    // work items, most costly first (cost = stages x tile width; ties keep
+    // member and tile order)
+    std::vector<std::pair<long, int2>> costed;
+    for (int m = 0; m < n; ++m) {
+        const MfmaPlan &p = plans[m];
+        for (int t = 0; t < p.nparts; ++t) {
+            const int xi = t % p.tiles_x;
+            const int nbx = xi < p.x_full ? 4 : (xi == p.x_full ? p.x_tail[0] : p.x_tail[1]);
+            costed.push_back({long(p.stages) * nbx, make_int2(m, t)});
+        }
+    }
+    std::stable_sort(costed.begin(), costed.end(),
+                     [](const std::pair<long, int2> &a, const std::pair<long, int2> &b) { return a.first > b.first; });
     std::vector<int2> items;
-    for (int m : order)
-        for (int t = 0; t < plans[m].nparts; ++t) items.push_back(make_int2(m, t));
+    items.reserve(costed.size());
+    for (const auto &c : costed) items.push_back(c.second);
 
     if (key != key_ || !exec_) {
         release();
@@ -968,7 +1020,9 @@ bool Ensemble::run(hipStream_t stream, int kernel, const std::vector<EnsembleMem
             d.tiles_k = p.tiles_k;
             d.tj_shift = p.tj_shift;
             d.tk = p.tk;
-            d.nbx = p.nbx;
+            d.x_full = p.x_full;
+            d.x_tail0 = p.x_tail[0];
+            d.x_tail1 = p.x_tail[1];
             d.part_base = part_base[m];
             d.scaling = members[m].scaling / p.zscale;
             d.sigma = members[m].sigma;
@@ -1281,22 +1335,32 @@ bool TileTuner::choose_saturated(const Geometry &g, int kernel, hipStream_t stre
             }
         }
     }
-    // the member's candidates: padded work / saturated rate
+    // the member's column tile: over its mixed x tiles, the padded work of each
+    // tile width over that width's saturated rate
     double best = -1.0;
-    for (int nbx = 4; nbx >= 2; --nbx) {
-        for (int tk = 8; tk >= 1; tk /= 2) {
-            const auto it = rate_.find({long(kernel), ks, long(nbx), long(tk)});
-            if (it == rate_.end()) continue;
-            MfmaPlan p;
-            if (!mfma_plan(g, np_, kernel, p, err, nbx, tk)) return false;
-            if (p.nbx != nbx || p.tk != tk) continue;
-            const double padded = double(p.blocks) * kTileC * 32.0 * nbx;  // points incl. padding
-            const double cost = padded / it->second;
-            if (best < 0.0 || cost < best) {
-                best = cost;
-                out.nbx = nbx;
-                out.tk = tk;
+    for (int tk = 8; tk >= 1; tk /= 2) {
+        MfmaPlan p;
+        if (!mfma_plan_mixed(g, np_, kernel, p, err, tk)) return false;
+        if (p.tk != tk) continue;
+        const long col_tiles = long(p.tiles_j) * p.tiles_k;
+        double cost = 0.0;
+        bool have = true;
+        const int widths[3] = {4, p.x_tail[0], p.x_tail[1]};
+        const int counts[3] = {p.x_full, p.x_tail[0] > 0, p.x_tail[1] > 0};
+        for (int w = 0; w < 3; ++w) {
+            if (counts[w] == 0) continue;
+            const auto it = rate_.find({long(kernel), ks, long(widths[w]), long(tk)});
+            if (it == rate_.end()) {
+                have = false;
+                break;
             }
+            cost += double(counts[w]) * col_tiles * kTileC * 32.0 * widths[w] / it->second;  // points incl. padding / rate
+        }
+        if (!have) continue;
+        if (best < 0.0 || cost < best) {
+            best = cost;
+            out.nbx = 0;
+            out.tk = tk;
         }
     }
     out.ms = best * np_;

@@ -36,6 +36,11 @@ struct MfmaPlan {
     int nparts = 0;         // statistics partials (one per block)
     size_t x_bytes = 0, y_bytes = 0, z_bytes = 0, table_bytes = 0;
     double zscale = 1.0;    // power of two applied to Z and undone in the scaling (fp16 range)
+    // mixed x tiles (ensemble members): x_full tiles of 128 points, then up
+    // to two tail tiles of 32 * x_tail[i] points (0: none); tiles_x counts all
+    bool mixed = false;
+    int x_full = 0;
+    int x_tail[2] = {0, 0};
 };
 
 // Power of two that keeps |Z Y| * 2^12 inside fp16 for the scaled-fp16
@@ -45,6 +50,9 @@ double mfma_zscale(int kernel, double amp_max);
 // nbx = 2..4 forces the x tile width, tk = 1, 2, 4 or 8 the column tile
 // (tj = 256 / tk); other values pick them by shape (least padding)
 bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int nbx = 0, int tk = 0);
+// The same with mixed x tiles: 128-point tiles and at most two tails of 64
+// or 96 points, which cover any multiple of 32 points without padding
+bool mfma_plan_mixed(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int tk = 0);
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err);
 // max_blocks > 0 launches only the first max_blocks tiles (the tile tuner's
@@ -62,8 +70,7 @@ struct EnsembleMember {
     int rescale = 0;
     double sigma = 0;
     void *d_field = nullptr;  // device output, `Real` elements
-    int nbx = 0;              // x tile override (0: by shape)
-    int tk = 0;               // column tile override (0: by shape)
+    int tk = 0;               // column tile override (0: by shape); x tiles are mixed
     double amp_max = 0;       // largest harmonic amplitude of the record
 };
 
@@ -93,11 +100,11 @@ public:
     // d_scratch: device room for g.points() values of Real (overwritten)
     template <class Real>
     bool choose(const Geometry &g, int kernel, Real *d_scratch, hipStream_t stream, TileChoice &out, std::string &err);
-    // Tile of a grid that runs inside a full launch (an ensemble member):
-    // the candidate with the least padded work per unit of its saturated
-    // throughput, the throughputs timed once per kernel and output type on
-    // a synthetic grid that fills the device with no padding for any
-    // candidate.
+    // Column tile (out.tk; out.nbx = 0) of a grid that runs inside a full
+    // launch with mixed x tiles (an ensemble member): the least padded work
+    // per unit of saturated throughput, summed over its x tiles, the
+    // throughputs timed once per kernel and output type on a synthetic grid
+    // that fills the device with no padding for any candidate.
     template <class Real>
     bool choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err);
     static bool forced(TileChoice &out);  // SCARF_TILE, when set to a valid tile
