@@ -320,8 +320,9 @@ __global__ __launch_bounds__(256) void direct_points(const double *__restrict__ 
             const double wx = sw[h][0], wy = sw[h][1], wz = sw[h][2], ph = sw[h][3];
 #pragma unroll
             for (int q = 0; q < kPtsP; ++q) {
-                double t = fma(wx, x[q], fma(wy, y[q], fma(wz, z[q], ph)));
-                t -= rint(t);
+                // phase in turns, reduced to [0, 1) by one v_fract_f64 (exact:
+                // t - floor(t)); cos(2 pi t) is the same for t - rint(t)
+                const double t = __builtin_amdgcn_fract(fma(wx, x[q], fma(wy, y[q], fma(wz, z[q], ph))));
                 if constexpr (F64) {
                     acc64[q] = fma(sa[h], cospi(2.0 * t), acc64[q]);
                 } else {
