@@ -159,6 +159,7 @@ private:
     hipGraphExec_t exec_ = nullptr;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     void *d_rec_ = nullptr, *d_tab_ = nullptr, *d_part_ = nullptr, *d_desc_ = nullptr, *d_items_ = nullptr;
+    void *d_first_ = nullptr;  // block b's items are d_items_[d_first_[b] .. d_first_[b + 1])
     void *d_stat_ = nullptr;
     void *d_targs_ = nullptr;  // per-member operand-table parameters
     size_t rec_bytes_ = 0;
