@@ -15,9 +15,13 @@ MEMBERS = [
     dict(fe=(64, 48, 33), ds=100.0, acf=0, cl=(4000.0, 3000.0, 2000.0), sigma=0.08, hurst=0.1, nharm=768, seed=3,
          alpha=20.0, rescale=1),
     dict(fe=(200, 40, 8), ds=20.0, acf=0, cl=(1500.0, 1500.0, 1500.0), sigma=0.05, hurst=0.4, nharm=256, seed=4),
-    # x tiles: 96 | 96 + 64 | 64 | 128 + 96 (ragged) | one 64-point tile for 20 points
     dict(fe=(20, 36, 12), ds=50.0, acf=0, cl=(1000.0, 1000.0, 1000.0), sigma=0.05, hurst=0.3, nharm=192, seed=5),
+    dict(fe=(180, 24, 8), ds=40.0, acf=1, cl=(600.0, 600.0, 200.0), sigma=0.1, nharm=256, seed=6),
+    dict(fe=(270, 16, 8), ds=30.0, acf=0, cl=(2500.0, 2500.0, 2500.0), sigma=0.05, hurst=0.2, nharm=256, seed=7),
+    dict(fe=(300, 16, 12), ds=30.0, acf=0, cl=(1200.0, 1200.0, 1200.0), sigma=0.05, hurst=0.15, nharm=224, seed=8),
 ]
+# x tiles (points): 96 | 96 + 64 | 64 | 128 + 96 (ragged) | one 64 for 20 | 128 + 64 (ragged) |
+# 128 + 96 + 64 (ragged) | 128 + 128 + 64 (ragged)
 
 
 def _cfg(m, real=np.float32):
