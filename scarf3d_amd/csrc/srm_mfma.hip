@@ -1305,15 +1305,20 @@ bool TileTuner::choose(const Geometry &g, int kernel, Real *d_scratch, hipStream
     auto padded = [&](const MfmaPlan &p) { return double(p.blocks) * kTileC * 32.0 * p.nbx; };
     TileChoice best;
     double best_ms = -1.0, def_ms = -1.0;
-    // Only the column shape is tuned, at the least-padding x width: every
-    // candidate then does the same work per block, so one wave of blocks
-    // compares them fairly. (Across x widths the first wave misranked them:
-    // at 1024 x 1024 x 512 it preferred the 96-wide tile, 15% slower in full.)
-    {
-        const int nbx = def.nbx;
+    // The column shape is tuned at the least-padding x width: every candidate
+    // then does the same work per block, so one wave of blocks compares them
+    // fairly. (Across x widths the first wave misranked them: at 1024 x 1024
+    // x 512 it preferred the 96-wide tile, 15% slower in full.) When that
+    // width leaves CUs without a block, narrower x tiles (more blocks) are
+    // tried too, as long as all their blocks still run at once: their timing
+    // then covers the whole grid.
+    for (int nbx = def.nbx; nbx >= 2; --nbx) {
+        if (nbx != def.nbx && def.blocks >= cus) break;
         for (int tk = 8; tk >= 1; tk /= 2) {
             MfmaPlan p;
             if (!mfma_plan(g, np_, kernel, p, err, nbx, tk)) return false;
+            if (p.nbx != nbx) continue;
+            if (nbx != def.nbx && p.blocks > cus) continue;
             const bool is_def = p.nbx == def.nbx && p.tk == def.tk;
             if (!is_def && padded(p) > 2.0 * padded(def)) continue;  // far more padding: cannot win
             if (!tune_reserve(&d_tab_, tab_cap_, p.table_bytes, err)) return false;
