@@ -1,4 +1,4 @@
-"""Full-size parity on the two large single-GPU configurations.
+"""Full-size parity on the benchmark configurations.
 
 512 x 512 x 512, 20,000 harmonics (the configuration the headline throughput
 is quoted on): the split-bf16 and scaled-fp16 MFMA fields against the FP64
@@ -8,11 +8,15 @@ against the field they were computed from.
 
 1024 x 1024 x 512, 50,000 harmonics, 16 points of interest with mute and
 taper, rescaled to sigma: the split-bf16 field against the FP64 kernel's, each
-after its own statistics, rescale and taper (bar 1e-5 sigma)."""
+after its own statistics, rescale and taper (bar 1e-5 sigma).
+
+128 x 128 x 128, 5,000 harmonics with the double library, and 2,000,000
+scattered points with 10,000 harmonics: every kernel against the FP64 kernel
+at every point, and the FP64 kernel against the oracle at 256 sampled points."""
 import numpy as np
 import pytest
 
-from srm_cases import init_product, structured_cfg
+from srm_cases import init_product, structured_cfg, unstructured_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +94,49 @@ def test_large_grid_with_taper_and_rescale_full_size(lib_f):
     assert err <= 1e-5
     # rescaled: stats(3) reports sigma and stats(4) zero for both
     assert st[2] == st64[2] and st[3] == st64[3] == 0.0
+
+
+def test_plain_128_double_library_full_size(oracle, lib_d):
+    n = (128, 128, 128)
+    cfg = structured_cfg(n=n, ds=50.0, acf=0, hurst=0.25, cl=(2000.0, 2000.0, 2000.0), sigma=0.05, nharm=5000,
+                         real=np.float64)
+    seed = 1234
+    print("[case] 128^3 x 5000, double library", flush=True)
+    init_product(lib_d, cfg)
+    ref, _ = _run(lib_d, "direct_f64", seed)
+    for kernel in ("direct_f32", "mfma_bf16x3", "mfma_f16x2"):
+        field, _ = _run(lib_d, kernel, seed)
+        err = _max_abs_diff(field, ref) / cfg["sigma"]
+        print(f"  {kernel} vs FP64 kernel, every point: max|err|/sigma = {err:.3e}", flush=True)
+        assert err <= 1e-5
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "d")
+    harm, used, _ = oracle.draw(p, s, seed, "d")
+    rng = np.random.default_rng(5)
+    ii, jj, kk = (rng.integers(0, n[a], 256) for a in range(3))
+    vals = oracle.field_at(p, s, harm, used, ii, jj, kk)
+    err64 = np.abs(_flat(ref)[ii + n[0] * (jj + n[1] * kk)] - vals).max() / cfg["sigma"]
+    print(f"  FP64 kernel vs oracle at 256 points: max|err|/sigma = {err64:.3e}", flush=True)
+    assert err64 <= 1e-11
+
+
+def test_points_configuration_full_size(oracle, lib_f):
+    cfg = unstructured_cfg(npts=2_000_000, box=(10000.0, 10000.0, 5000.0), dh=50.0, acf=1,
+                           cl=(3000.0, 3000.0, 1000.0), sigma=0.1, hurst=0.25, nharm=10000, seed=7, real=np.float32)
+    seed = 7
+    print("[case] 2,000,000 points x 10000", flush=True)
+    init_product(lib_f, cfg)
+    ref, _ = _run(lib_f, "direct_f64", seed)
+    field, _ = _run(lib_f, "direct_f32", seed)
+    err = _max_abs_diff(field, ref) / cfg["sigma"]
+    print(f"  direct_f32 vs FP64 kernel, every point: max|err|/sigma = {err:.3e}", flush=True)
+    assert err <= 1e-5
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "f")
+    harm, used, _ = oracle.draw(p, s, seed, "f")
+    flat = np.random.default_rng(5).integers(0, p.npts, 256)
+    sub = dict(cfg, x=cfg["x"][flat], y=cfg["y"][flat], z=cfg["z"][flat])  # same parameters, sampled points
+    vals = oracle.field_unstructured(oracle.params(sub), s, harm, used)
+    err64 = np.abs(_flat(ref)[flat].astype(np.float64) - vals).max() / cfg["sigma"]
+    print(f"  FP64 kernel vs oracle at 256 points: max|err|/sigma = {err64:.3e}", flush=True)
+    assert err64 <= 1e-6
