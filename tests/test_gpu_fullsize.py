@@ -12,7 +12,9 @@ after its own statistics, rescale and taper (bar 1e-5 sigma).
 
 128 x 128 x 128, 5,000 harmonics with the double library, and 2,000,000
 scattered points with 10,000 harmonics: every kernel against the FP64 kernel
-at every point, and the FP64 kernel against the oracle at 256 sampled points."""
+at every point, and the FP64 kernel against the oracle at 256 sampled points.
+
+The 64-member ensemble: every member equal to its single run bit for bit."""
 import numpy as np
 import pytest
 
@@ -140,3 +142,30 @@ def test_points_configuration_full_size(oracle, lib_f):
     err64 = np.abs(_flat(ref)[flat].astype(np.float64) - vals).max() / cfg["sigma"]
     print(f"  FP64 kernel vs oracle at 256 points: max|err|/sigma = {err64:.3e}", flush=True)
     assert err64 <= 1e-6
+
+
+def test_ensemble_64_full_size(lib_f):
+    """The 64-member benchmark ensemble (grids 96^3 .. 384 x 384 x 192, 2k-16k
+    harmonics; one persistent launch over mixed x tiles, items dealt to the
+    least-loaded block): every member's field and statistics equal its single
+    run bit for bit."""
+    import bench
+    from scarf3d_amd.hip import Hip
+
+    hip = Hip()
+    members = bench.ensemble_members()
+    bufs = [hip.empty(int(np.prod(m["fe"])), np.float32) for m in members]
+    print("[case] ensemble of 64, mfma_bf16x3", flush=True)
+    stats = lib_f.ensemble_execute(members, [b.ptr for b in bufs], "mfma_bf16x3")
+    for i, m in enumerate(members):
+        cfg = structured_cfg(n=m["fe"], ds=m["ds"], acf=m["acf"], cl=m["cl"], sigma=m["sigma"],
+                             hurst=m.get("hurst", 0.0), nharm=m["nharm"], real=np.float32)
+        init_product(lib_f, cfg)
+        assert lib_f.set_kernel("mfma_bf16x3") == 0
+        single, st = lib_f.execute(m["seed"])
+        lib_f.finalize()
+        assert np.array_equal(bufs[i].numpy(), _flat(single)), i
+        assert np.array_equal(stats[i, :4], st[:4]), i
+    print(f"  {len(members)} members equal their single runs", flush=True)
+    for b in bufs:
+        b.free()
