@@ -73,6 +73,58 @@ int fail(const std::string &msg)
 
 constexpr bool kDouble = sizeof(real) == sizeof(double);
 
+// The field back into the caller's array (pageable memory): chunks go by DMA
+// into two pinned staging buffers and from there, on up to 8 threads, into the
+// array, the DMA of chunk i+1 running while chunk i is copied out. One
+// hipMemcpy into pageable memory moved 537 MB in 52-63 ms. The staging buffers
+// are allocated on the first large copy and kept for the process.
+constexpr size_t kStageBytes = size_t(32) << 20;
+
+bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, std::string &err)
+{
+    auto ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        err = std::string("copying the field to the host: ") + what + ": " + hipGetErrorString(e);
+        return false;
+    };
+    if (bytes <= kStageBytes) return ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    static char *stage = nullptr;
+    static hipEvent_t done[2] = {nullptr, nullptr};
+    if (!stage) {
+        void *p = nullptr;
+        if (!ok(hipHostMalloc(&p, 2 * kStageBytes, hipHostMallocDefault), "hipHostMalloc")) return false;
+        for (auto &e : done)
+            if (!ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) return false;
+        stage = static_cast<char *>(p);
+    }
+    const size_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
+    auto len = [&](size_t i) { return std::min(kStageBytes, bytes - i * kStageBytes); };
+    auto fetch = [&](size_t i) {
+        return ok(hipMemcpyAsync(stage + (i % 2) * kStageBytes, static_cast<const char *>(src) + i * kStageBytes,
+                                 len(i), hipMemcpyDeviceToHost, stream),
+                  "hipMemcpyAsync") &&
+               ok(hipEventRecord(done[i % 2], stream), "hipEventRecord");
+    };
+    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    auto drain = [&](size_t i) {  // chunk i: staging buffer i % 2 -> the caller's array
+        if (!ok(hipEventSynchronize(done[i % 2]), "hipEventSynchronize")) return false;
+        char *out = static_cast<char *>(dst) + i * kStageBytes;
+        const char *in = stage + (i % 2) * kStageBytes;
+        const size_t n = len(i), part = (n + nt - 1) / nt;
+        std::vector<std::thread> workers;
+        for (unsigned t = 1; t < nt && t * part < n; ++t)
+            workers.emplace_back([=] { std::memcpy(out + t * part, in + t * part, std::min(part, n - t * part)); });
+        std::memcpy(out, in, std::min(part, n));
+        for (auto &w : workers) w.join();
+        return true;
+    };
+    // buffer i % 2 is refilled only after chunk i - 2 has been drained from it
+    if (!fetch(0)) return false;
+    for (size_t i = 1; i < nchunks; ++i)
+        if (!fetch(i) || !drain(i - 1)) return false;
+    return drain(nchunks - 1);
+}
+
 int kernel_from_env()
 {
     const char *e = std::getenv("SCARF_KERNEL");
@@ -402,8 +454,7 @@ void scarf_execute(const int seed, real *field, real stats[])
         return;
     }
     if (run(static_cast<real *>(d), stats) != 0) return;
-    const hipError_t e = hipMemcpy(field, d, size_t(n) * sizeof(real), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) fail(std::string("copying the field to the host: ") + hipGetErrorString(e));
+    if (!copy_to_host(field, d, size_t(n) * sizeof(real), s.device->stream(), err)) fail(err);
 }
 
 void scarf_finalize(void)

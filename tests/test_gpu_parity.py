@@ -217,3 +217,20 @@ def test_every_forced_tile_gives_the_same_field(lib_f, monkeypatch, kernel):
     lib_f.finalize()
     field, _ = _run(lib_f, cfg, 5, kernel)
     assert np.abs(field.astype(np.float64) - ref).max() <= 1e-7 * cfg["sigma"]
+
+
+def test_host_copy_equals_device_field(lib_f):
+    """scarf_execute stages fields above 32 MiB through pinned buffers; the
+    host array equals the device field bit for bit (72 MB: two whole 32 MiB
+    chunks and a partial one)."""
+    from scarf3d_amd.hip import Hip
+
+    hip = Hip()
+    n = (300, 300, 200)
+    cfg = structured_cfg(n=n, nharm=64, real=np.float32)
+    _say("host copy of a 72 MB field", n)
+    field, _ = _run(lib_f, cfg, 11, "mfma_bf16x3")
+    d = hip.empty(int(np.prod(n)), np.float32)
+    lib_f.execute_device(11, d.ptr)
+    assert np.array_equal(field.reshape(-1, order="F"), d.numpy())
+    d.free()
