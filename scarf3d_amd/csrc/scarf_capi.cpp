@@ -77,7 +77,8 @@ constexpr bool kDouble = sizeof(real) == sizeof(double);
 // into two pinned staging buffers and from there, on up to 8 threads, into the
 // array, the DMA of chunk i+1 running while chunk i is copied out. One
 // hipMemcpy into pageable memory moved 537 MB in 52-63 ms. The staging buffers
-// are allocated on the first large copy and kept for the process.
+// are allocated on the first large copy and kept for the process; if pinned
+// memory is refused, every copy is one hipMemcpy.
 constexpr size_t kStageBytes = size_t(32) << 20;
 
 bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, std::string &err)
@@ -87,16 +88,21 @@ bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, 
         err = std::string("copying the field to the host: ") + what + ": " + hipGetErrorString(e);
         return false;
     };
-    if (bytes <= kStageBytes) return ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
     static char *stage = nullptr;
     static hipEvent_t done[2] = {nullptr, nullptr};
-    if (!stage) {
+    static bool no_stage = false;  // pinned memory was refused once: one copy from then on
+    if (!stage && !no_stage && bytes > kStageBytes) {
         void *p = nullptr;
-        if (!ok(hipHostMalloc(&p, 2 * kStageBytes, hipHostMallocDefault), "hipHostMalloc")) return false;
-        for (auto &e : done)
-            if (!ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) return false;
-        stage = static_cast<char *>(p);
+        if (hipHostMalloc(&p, 2 * kStageBytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            no_stage = true;
+        } else {
+            for (auto &e : done)
+                if (!ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) return false;
+            stage = static_cast<char *>(p);
+        }
     }
+    if (bytes <= kStageBytes || !stage) return ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
     const size_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
     auto len = [&](size_t i) { return std::min(kStageBytes, bytes - i * kStageBytes); };
     auto fetch = [&](size_t i) {
