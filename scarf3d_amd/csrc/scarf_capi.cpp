@@ -241,7 +241,7 @@ int prepare(int seed)
 }
 
 // device half: tables, field, statistics, epilogue into d_field
-int run(real *d_field, real stats[])
+int run(real *d_field, real stats[], const scarf::Device::SlabHook *on_slab = nullptr)
 {
     State &s = state();
     if (!s.table_ready) return fail("no harmonic table drawn (call scarf_srm_draw first)");
@@ -250,7 +250,7 @@ int run(real *d_field, real stats[])
     const int kernel = resolve_kernel(s.cfg);
     double st[6] = {};
     std::string err;
-    if (!s.device->run<real>(g, kernel, s.setup.scaling, ep, d_field, st, err)) return fail(err);
+    if (!s.device->run<real>(g, kernel, s.setup.scaling, ep, d_field, st, err, on_slab)) return fail(err);
     if (stats) {
         clear_stats(stats);
         stats[0] = static_cast<real>(s.setup.info1);
@@ -459,8 +459,24 @@ void scarf_execute(const int seed, real *field, real stats[])
         fail(err);
         return;
     }
-    if (run(static_cast<real *>(d), stats) != 0) return;
-    if (!copy_to_host(field, d, size_t(n) * sizeof(real), s.device->stream(), err)) fail(err);
+    // Fields above 64 MiB come back in z slabs, each copied (on a stream of
+    // its own) while the slabs after it are computed; the device's run hands
+    // the slabs over when they are final, or the whole field at the end.
+    static hipStream_t copy_stream = nullptr;
+    const scarf::Device::SlabHook to_host = [&](long first, long count, std::string &e) {
+        if (!copy_stream) {
+            const hipError_t rc = hipStreamCreate(&copy_stream);  // default priority
+            if (rc != hipSuccess) {
+                e = std::string("hipStreamCreate: ") + hipGetErrorString(rc);
+                return false;
+            }
+        }
+        return copy_to_host(field + first, static_cast<const real *>(d) + first, size_t(count) * sizeof(real),
+                            copy_stream, e);
+    };
+    const bool slabs = size_t(n) * sizeof(real) > 2 * kStageBytes;
+    if (run(static_cast<real *>(d), stats, slabs ? &to_host : nullptr) != 0) return;
+    if (!slabs && !copy_to_host(field, d, size_t(n) * sizeof(real), s.device->stream(), err)) fail(err);
 }
 
 void scarf_finalize(void)

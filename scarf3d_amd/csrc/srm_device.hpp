@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -69,9 +70,14 @@ public:
     // Runs tables + field + statistics (+ epilogue) into d_field (device,
     // `Real` elements). stats_out[0..3] = {count, mean, variance, sigma_d}
     // in FP64 and real_stats[0..1] = {info3, info4} rounded to Real.
+    // on_slab, if given, is called with (first point, points) for each part
+    // of the field as soon as it is final: in z order while the rest is still
+    // computed (MFMA grids with no rescale or taper), else once for the whole
+    // field at the end.
+    using SlabHook = std::function<bool(long, long, std::string &)>;
     template <class Real>
     bool run(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
-             double *stats_out, std::string &err);
+             double *stats_out, std::string &err, const SlabHook *on_slab = nullptr);
 
     hipStream_t stream() const { return stream_; }
     const Timings &timings() const { return timings_; }
@@ -93,6 +99,8 @@ private:
     int device_ = -1;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[6] = {};
+    static constexpr int kSlabs = 8;  // field parts handed out while the rest is computed
+    hipEvent_t slab_ev_[kSlabs] = {};
     int nharm_ = 0;
     double amp_max_ = 0;  // largest amplitude of the resident table
     std::unique_ptr<TileTuner> tuner_;

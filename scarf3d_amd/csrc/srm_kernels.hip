@@ -467,6 +467,8 @@ Device::~Device()
     if (h_stat_) hipHostFree(h_stat_);
     for (auto &e : ev_)
         if (e) hipEventDestroy(e);
+    for (auto &e : slab_ev_)
+        if (e) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
 }
 
@@ -554,7 +556,7 @@ bool Device::upload_points(const Real *x, const Real *y, const Real *z, long n, 
 
 template <class Real>
 bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
-                 double *stats_out, std::string &err)
+                 double *stats_out, std::string &err, const SlabHook *on_slab)
 {
     HIP_TRY(hipSetDevice(device_));
     if (!d_harm_ || nharm_ <= 0) {
@@ -577,6 +579,10 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
     HIP_TRY(hipEventRecord(ev_[0], stream_));
 
     int nparts = 0;
+    // field parts (first point, points, event) handed to on_slab while the
+    // rest is computed: z slabs of whole k-tile rows, MFMA grids only, and
+    // only when no rescale or taper changes the field afterwards
+    std::vector<long> slab_first, slab_points;
     if (g.structured) {
         if (mfma) {
             MfmaPlan plan;
@@ -593,7 +599,26 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
             d_part_ = static_cast<double *>(q);
             if (!mfma_tables(plan, g, d_harm_, np_, d_tab_, stream_, err)) return false;
             HIP_TRY(hipEventRecord(ev_[1], stream_));
-            if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err)) return false;
+            if (on_slab && !ep.active() && plan.tiles_k > 1) {
+                const long row = long(plan.tiles_x) * plan.tiles_j;  // tiles of one k-tile row
+                const int nslab = std::min(kSlabs, plan.tiles_k);
+                const int per = (plan.tiles_k + nslab - 1) / nslab;
+                const long plane = long(g.nx) * g.ny;
+                for (int k0 = 0; k0 < plan.tiles_k; k0 += per) {
+                    const int k1 = std::min(k0 + per, plan.tiles_k);
+                    const int c = int(slab_first.size());
+                    if (!slab_ev_[c]) HIP_TRY(hipEventCreateWithFlags(&slab_ev_[c], hipEventDisableTiming));
+                    if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err, (k1 - k0) * row,
+                                    k0 * row))
+                        return false;
+                    HIP_TRY(hipEventRecord(slab_ev_[c], stream_));
+                    const long z0 = long(k0) * plan.tk, z1 = std::min<long>(long(k1) * plan.tk, g.nz);
+                    slab_first.push_back(z0 * plane);
+                    slab_points.push_back((z1 - z0) * plane);
+                }
+            } else if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err)) {
+                return false;
+            }
         } else {
             const bool f64 = kernel == kKernelDirectF64;
             const size_t elt = f64 ? sizeof(double) : sizeof(float);
@@ -682,7 +707,12 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
     }
     HIP_TRY(hipEventRecord(ev_[4], stream_));
     HIP_TRY(hipMemcpyAsync(h_stat_, d_stat_, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    for (size_t c = 0; c < slab_first.size(); ++c) {
+        HIP_TRY(hipEventSynchronize(slab_ev_[c]));
+        if (!(*on_slab)(slab_first[c], slab_points[c], err)) return false;
+    }
     HIP_TRY(hipStreamSynchronize(stream_));
+    if (on_slab && slab_first.empty() && !(*on_slab)(0, npts, err)) return false;
     float ms = 0.f;
     hipEventElapsedTime(&ms, ev_[0], ev_[1]);
     timings_.tables = ms;
@@ -700,8 +730,9 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
 
 template bool Device::upload_points<float>(const float *, const float *, const float *, long, std::string &);
 template bool Device::upload_points<double>(const double *, const double *, const double *, long, std::string &);
-template bool Device::run<float>(const Geometry &, int, double, const Epilogue &, float *, double *, std::string &);
+template bool Device::run<float>(const Geometry &, int, double, const Epilogue &, float *, double *, std::string &,
+                                 const SlabHook *);
 template bool Device::run<double>(const Geometry &, int, double, const Epilogue &, double *, double *,
-                                  std::string &);
+                                  std::string &, const SlabHook *);
 
 }  // namespace scarf

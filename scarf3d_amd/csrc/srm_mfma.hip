@@ -576,10 +576,10 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
                                                          const float2 *__restrict__ zt, int nx, int ny, int nz,
                                                          int nx_pad, int stages, int tiles_x, int tj_shift, int tk,
                                                          int tiles_j, double scaling, Out *__restrict__ field,
-                                                         double *__restrict__ part)
+                                                         double *__restrict__ part, long first)
 {
     __shared__ StageLds<SchemeOf<S>::parts> lds0, lds1;
-    const long b = blockIdx.x;
+    const long b = first + blockIdx.x;
     const int x0 = int(b % tiles_x) * 32 * NBX;
     const long ct = b / tiles_x;
     const int j0 = int(ct % tiles_j) << tj_shift;
@@ -883,7 +883,7 @@ namespace {
 
 template <int S, class Real>
 void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, double scaling, Real *d_field,
-                 double *d_part, hipStream_t stream, long blocks)
+                 double *d_part, hipStream_t stream, long blocks, long first)
 {
     using E = typename SchemeOf<S>::elt;
     const E *xt = reinterpret_cast<const E *>(base);
@@ -891,7 +891,7 @@ void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, doub
     const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes);
     const dim3 grid(static_cast<unsigned>(blocks));
 #define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x, plan.tj_shift, plan.tk, \
-                        plan.tiles_j, scaling, d_field, d_part
+                        plan.tiles_j, scaling, d_field, d_part, first
     switch (plan.nbx) {
     case 2: mfma_grid<S, 2, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
     case 3: mfma_grid<S, 3, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
@@ -904,15 +904,21 @@ void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, doub
 
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err, long max_blocks)
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks, long first_block)
 {
     const char *base = static_cast<const char *>(d_tab);
     scaling /= plan.zscale;  // exact: a power of two
-    const long blocks = max_blocks > 0 ? std::min(plan.blocks, max_blocks) : plan.blocks;
+    if (first_block < 0 || first_block >= plan.blocks) {
+        err = "mfma_field: first tile out of range";
+        return false;
+    }
+    const long left = plan.blocks - first_block;
+    const long blocks = max_blocks > 0 ? std::min(left, max_blocks) : left;
+    const long f = first_block;
     switch (scheme_of(plan.kernel)) {
-    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
-    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
-    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks); break;
+    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
+    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
+    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -923,7 +929,7 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
 }
 
 template bool mfma_field<float>(const MfmaPlan &, const Geometry &, const void *, double, float *, double *,
-                                hipStream_t, std::string &, long);
+                                hipStream_t, std::string &, long, long);
 
 // ---------------------------------------------------------------------------
 Ensemble::~Ensemble() { release(); }
@@ -1442,6 +1448,6 @@ template bool TileTuner::choose<float>(const Geometry &, int, float *, hipStream
 template bool TileTuner::choose<double>(const Geometry &, int, double *, hipStream_t, TileChoice &, std::string &);
 
 template bool mfma_field<double>(const MfmaPlan &, const Geometry &, const void *, double, double *, double *,
-                                 hipStream_t, std::string &, long);
+                                 hipStream_t, std::string &, long, long);
 
 }  // namespace scarf

@@ -55,11 +55,12 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
 bool mfma_plan_mixed(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int tk = 0);
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err);
-// max_blocks > 0 launches only the first max_blocks tiles (the tile tuner's
-// timing runs); the field is then incomplete
+// Launches tiles first_block .. first_block + max_blocks - 1 (all from
+// first_block on when max_blocks is 0). Fewer than all tiles leave the field
+// incomplete: the tile tuner's timing runs, or a field computed in parts.
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err, long max_blocks = 0);
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks = 0, long first_block = 0);
 
 // One realization of an ensemble, prepared on the host.
 struct EnsembleMember {
