@@ -127,3 +127,30 @@ void orc_baseline_unstructured_f(const orc_params *p, const orc_setup *s, const 
         for (long q = pb; q < pe; ++q) field[q - i0] = field[q - i0] * scaling;
     }
 }
+
+/* The same at double precision (the DOUBLE_PREC build's loop). */
+void orc_baseline_unstructured_d(const orc_params *p, const orc_setup *s, const double *harm,
+                                 const uint8_t *used, long i0, long i1, double *field, int threads)
+{
+    const double *m = s->matrix, *bar = s->bar;
+    const double scaling = s->scaling;
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+    for (int t = 0; t < threads; ++t) {
+        const long span = i1 - i0;
+        const long pb = i0 + span * t / threads, pe = i0 + span * (t + 1) / threads;
+        for (int l = 0; l < p->nharm; ++l) {
+            if (!used[l]) continue;
+            const double *h = harm + 5 * l;
+            const double v1 = h[0], v2 = h[1], v3 = h[2], a = h[3], b = h[4];
+            for (long q = pb; q < pe; ++q) {
+                const double x = p->x[q], y = p->y[q], z = p->z[q];
+                const double arg = v1 * ((m[0] * x + m[3] * y + m[6] * z) + bar[0]) +
+                                   v2 * ((m[1] * x + m[4] * y + m[7] * z) + bar[1]) +
+                                   v3 * ((m[2] * x + m[5] * y + m[8] * z) + bar[2]);
+                field[q - i0] = field[q - i0] + a * sin(arg) + b * cos(arg);
+            }
+        }
+        for (long q = pb; q < pe; ++q) field[q - i0] = field[q - i0] * scaling;
+    }
+}

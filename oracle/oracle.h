@@ -127,9 +127,11 @@ void orc_baseline_structured_f(const orc_params *p, const orc_setup *s, const do
                                const uint8_t *used, int k0, int k1, float *field, int threads);
 void orc_baseline_structured_d(const orc_params *p, const orc_setup *s, const double *harm,
                                const uint8_t *used, int k0, int k1, double *field, int threads);
-/* the same on a point list: points [i0, i1), float precision */
+/* the same on a point list: points [i0, i1), float or double precision */
 void orc_baseline_unstructured_f(const orc_params *p, const orc_setup *s, const double *harm,
                                  const uint8_t *used, long i0, long i1, float *field, int threads);
+void orc_baseline_unstructured_d(const orc_params *p, const orc_setup *s, const double *harm,
+                                 const uint8_t *used, long i0, long i1, double *field, int threads);
 
 #ifdef __cplusplus
 }

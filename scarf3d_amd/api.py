@@ -184,9 +184,18 @@ class Library:
         self.grid = (len(xa),)
         self.npts = len(xa)
 
-    def execute(self, seed: int):
-        """scarf_execute: returns (field in Fortran order of the local block, stats[8])."""
-        field = np.zeros(self.npts, dtype=self.dtype)
+    def execute(self, seed: int, out: Optional[np.ndarray] = None):
+        """scarf_execute: returns (field in Fortran order of the local block,
+        stats[8]). `out`, a contiguous array of npts `real` values, receives
+        the field instead of a new array (a caller's field array reused
+        across realizations)."""
+        if out is None:
+            field = np.zeros(self.npts, dtype=self.dtype)
+        else:
+            field = out.reshape(-1, order="A")
+            if field.dtype != self.dtype or field.size != self.npts or not (out.flags.c_contiguous or
+                                                                             out.flags.f_contiguous):
+                raise ValueError("out must be a contiguous array of npts real values")
         stats = np.zeros(8, dtype=self.dtype)
         self.lib.scarf_execute(seed, field.ctypes.data_as(C.c_void_p), self._rp(stats))
         return field.reshape(self.grid, order="F"), stats

@@ -117,20 +117,6 @@ def build_standalone(verbose: bool = False) -> list[Path]:
     return outs
 
 
-def build_ceiling_probe(verbose: bool = False) -> list[Path]:
-    """mfma_ceiling (csrc/mfma_ceiling.hip): the sustained MFMA rate of the
-    device with constant and with changing operands, the ceiling the MFMA
-    field kernels are compared with."""
-    exe = LIB / "mfma_ceiling"
-    cmd = [HIPCC, "-x", "hip", "-std=c++17", "-Wall", *DEVICE, str(CSRC / "mfma_ceiling.hip"), "-o", str(exe)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"mfma_ceiling build failed:\n{res.stdout}{res.stderr}")
-    return [exe]
-
-
 def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
     """Compile every HIP translation unit for gfx950 and link both libraries."""
     BUILD.mkdir(exist_ok=True)
@@ -149,7 +135,6 @@ def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
         outs.append(out)
     outs.extend(build_standalone(verbose))
     outs.extend(build_fortran(verbose))
-    outs.extend(build_ceiling_probe(verbose))
     return outs
 
 

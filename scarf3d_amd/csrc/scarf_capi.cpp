@@ -29,7 +29,9 @@
 #include "srm_device.hpp"
 #include "srm_host.hpp"
 #include "srm_mfma.hpp"
+#include "worker_pool.hpp"
 
+#include <climits>
 #include <thread>
 
 namespace {
@@ -44,6 +46,9 @@ struct State {
     int nharm = 10000;
     int draw_threads = 0;  // host draw workers (0: default)
     long draw_block = 0;   // pairs per draw block (0: default)
+    bool stream_draw = true;  // MFMA grids: device work starts on the first drawn harmonics
+    int draw_chunk = 0;       // scarf_draw_harmonics: kept harmonics per cut of the walk (0: as streamed)
+    int stream_chunk = 0;     // kept harmonics per streamed chunk, a multiple of 128 (0: growing chunks)
     std::string error;
     std::unique_ptr<scarf::Device> device;
     std::unique_ptr<scarf::Ensemble> ensemble;
@@ -74,12 +79,22 @@ int fail(const std::string &msg)
 constexpr bool kDouble = sizeof(real) == sizeof(double);
 
 // The field back into the caller's array (pageable memory): chunks go by DMA
-// into two pinned staging buffers and from there, on up to 8 threads, into the
-// array, the DMA of chunk i+1 running while chunk i is copied out. One
-// hipMemcpy into pageable memory moved 537 MB in 52-63 ms. The staging buffers
-// are allocated on the first large copy and kept for the process; if pinned
+// into two pinned staging buffers and from there, on up to 8 pool threads,
+// into the array, the DMA of chunk i+1 running while chunk i is copied out.
+// One hipMemcpy into pageable memory moved 537 MB in 52-63 ms (~9 GB/s at
+// any size). Copies of a few MiB and more are staged, in at least four
+// chunks of 4 to 32 MiB so that DMA and drain overlap. The staging buffers
+// are allocated on the first staged copy and kept for the process; if pinned
 // memory is refused, every copy is one hipMemcpy.
-constexpr size_t kStageBytes = size_t(32) << 20;
+constexpr size_t kStageBytes = size_t(32) << 20;  // staging buffer size, largest chunk
+constexpr size_t kStageMin = size_t(4) << 20;     // smallest chunk and smallest staged copy
+
+scarf::WorkerPool &drain_pool()
+{
+    static scarf::WorkerPool *p = new scarf::WorkerPool(
+        static_cast<int>(std::max(1u, std::min(8u, std::thread::hardware_concurrency()))));  // lives until exit
+    return *p;
+}
 
 bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, std::string &err)
 {
@@ -91,7 +106,7 @@ bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, 
     static char *stage = nullptr;
     static hipEvent_t done[2] = {nullptr, nullptr};
     static bool no_stage = false;  // pinned memory was refused once: one copy from then on
-    if (!stage && !no_stage && bytes > kStageBytes) {
+    if (!stage && !no_stage && bytes >= kStageMin) {
         void *p = nullptr;
         if (hipHostMalloc(&p, 2 * kStageBytes, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
@@ -102,26 +117,27 @@ bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, 
             stage = static_cast<char *>(p);
         }
     }
-    if (bytes <= kStageBytes || !stage) return ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
-    const size_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
-    auto len = [&](size_t i) { return std::min(kStageBytes, bytes - i * kStageBytes); };
+    if (bytes < kStageMin || !stage) return ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    const size_t chunk = std::min(kStageBytes, std::max(kStageMin, (bytes / 4 + 4095) / 4096 * 4096));
+    const size_t nchunks = (bytes + chunk - 1) / chunk;
+    auto len = [&](size_t i) { return std::min(chunk, bytes - i * chunk); };
     auto fetch = [&](size_t i) {
-        return ok(hipMemcpyAsync(stage + (i % 2) * kStageBytes, static_cast<const char *>(src) + i * kStageBytes,
-                                 len(i), hipMemcpyDeviceToHost, stream),
+        return ok(hipMemcpyAsync(stage + (i % 2) * kStageBytes, static_cast<const char *>(src) + i * chunk, len(i),
+                                 hipMemcpyDeviceToHost, stream),
                   "hipMemcpyAsync") &&
                ok(hipEventRecord(done[i % 2], stream), "hipEventRecord");
     };
-    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    scarf::WorkerPool &pool = drain_pool();
     auto drain = [&](size_t i) {  // chunk i: staging buffer i % 2 -> the caller's array
         if (!ok(hipEventSynchronize(done[i % 2]), "hipEventSynchronize")) return false;
-        char *out = static_cast<char *>(dst) + i * kStageBytes;
+        char *out = static_cast<char *>(dst) + i * chunk;
         const char *in = stage + (i % 2) * kStageBytes;
-        const size_t n = len(i), part = (n + nt - 1) / nt;
-        std::vector<std::thread> workers;
-        for (unsigned t = 1; t < nt && t * part < n; ++t)
-            workers.emplace_back([=] { std::memcpy(out + t * part, in + t * part, std::min(part, n - t * part)); });
-        std::memcpy(out, in, std::min(part, n));
-        for (auto &w : workers) w.join();
+        const size_t n = len(i), nt = size_t(pool.size()), part = (n + nt - 1) / nt;
+        const std::function<void(int)> job = [&](int t) {
+            const size_t lo = size_t(t) * part;
+            if (lo < n) std::memcpy(out + lo, in + lo, std::min(part, n - lo));
+        };
+        pool.run(job);
         return true;
     };
     // buffer i % 2 is refilled only after chunk i - 2 has been drained from it
@@ -240,6 +256,22 @@ int prepare(int seed)
     return d.n_used;
 }
 
+// stats(1:8) of a finished realization (src/scarflib.f90:363,377)
+void report_stats(real stats[], const double st[6])
+{
+    State &s = state();
+    if (stats) {
+        clear_stats(stats);
+        stats[0] = static_cast<real>(s.setup.info1);
+        stats[1] = static_cast<real>(s.setup.info2);
+        // rescale reports sigma and zero mean (src/scarflib_srm.f90:909-910)
+        stats[2] = s.cfg.rescale == 1 ? static_cast<real>(s.cfg.sigma) : static_cast<real>(st[3]);
+        stats[3] = s.cfg.rescale == 1 ? real(0) : static_cast<real>(st[4]);
+        stats[4] = static_cast<real>(s.host_seconds);
+        stats[5] = static_cast<real>(s.device->timings().total * 1e-3);
+    }
+}
+
 // device half: tables, field, statistics, epilogue into d_field
 int run(real *d_field, real stats[], const scarf::Device::SlabHook *on_slab = nullptr)
 {
@@ -251,16 +283,68 @@ int run(real *d_field, real stats[], const scarf::Device::SlabHook *on_slab = nu
     double st[6] = {};
     std::string err;
     if (!s.device->run<real>(g, kernel, s.setup.scaling, ep, d_field, st, err, on_slab)) return fail(err);
-    if (stats) {
-        clear_stats(stats);
-        stats[0] = static_cast<real>(s.setup.info1);
-        stats[1] = static_cast<real>(s.setup.info2);
-        // rescale reports sigma and zero mean (src/scarflib_srm.f90:909-910)
-        stats[2] = s.cfg.rescale == 1 ? static_cast<real>(s.cfg.sigma) : static_cast<real>(st[3]);
-        stats[3] = s.cfg.rescale == 1 ? real(0) : static_cast<real>(st[4]);
-        stats[4] = static_cast<real>(s.host_seconds);
-        stats[5] = static_cast<real>(s.device->timings().total * 1e-3);
+    report_stats(stats, st);
+    s.error.clear();
+    return 0;
+}
+
+// One realization into d_field. On the MFMA kernels of a structured grid
+// the draw is streamed: the device starts on the first chunk of harmonics
+// while the host draws the next (the reference draws each harmonic before
+// its device loop, src/scarflib_srm.f90:725-833, so draw and device work
+// never overlap there). Elsewhere the whole table is drawn first.
+int execute_realization(int seed, real *d_field, real stats[], const scarf::Device::SlabHook *on_slab)
+{
+    State &s = state();
+    s.table_ready = false;
+    if (!s.initialized) return fail("execute called before initialize");
+    if (s.cfg.method != 1) return fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
+    s.cfg.nharm = s.nharm;
+    const int kernel = resolve_kernel(s.cfg);
+    const bool mfma = kernel == scarf::kKernelMfmaBf16x3 || kernel == scarf::kKernelMfmaF16x2 ||
+                      kernel == scarf::kKernelMfmaBf16x2;
+    if (!s.cfg.structured || !mfma || !s.stream_draw) {
+        if (prepare(seed) < 0) return -1;
+        return run(d_field, stats, on_slab);
     }
+    std::string err;
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, err) : scarf::srm_setup<float>(s.cfg, s.setup, err);
+    if (rc != 0) return fail(err);
+    double host = std::chrono::duration<double>(clock::now() - t0).count();
+    if (!ensure_device()) return -1;
+    std::unique_ptr<scarf::HarmonicSampler> sampler =
+        kDouble ? scarf::HarmonicSampler::create<double>(s.cfg, s.setup, seed, s.draw_threads, s.draw_block)
+                : scarf::HarmonicSampler::create<float>(s.cfg, s.setup, seed, s.draw_threads, s.draw_block);
+    const scarf::Device::HarmonicFeed feed = [&](int kept, scarf::HarmonicTable &t, bool &last, std::string &) {
+        const auto a = clock::now();
+        const int l0 = sampler->position();
+        sampler->draw_until(kept);
+        scarf::append_table(s.setup, sampler->draw(), l0, sampler->position(), t);
+        last = sampler->finished();
+        host += std::chrono::duration<double>(clock::now() - a).count();
+        return true;
+    };
+    const scarf::Geometry g = geometry(s.cfg);
+    const scarf::Epilogue ep = epilogue(s.cfg, s.setup.ndim);
+    double st[6] = {};
+    bool restart = false;
+    if (!s.device->run_streamed<real>(g, kernel, s.cfg.nharm, s.setup.scaling, ep, d_field, st, err, feed, &restart,
+                                      on_slab, s.stream_chunk)) {
+        if (!restart) return fail(err);
+        // an amplitude the scaled-fp16 table cannot hold unscaled: the rest of
+        // the draw, then the whole table with its power-of-two scale
+        const auto a = clock::now();
+        sampler->draw_until(INT_MAX);
+        const scarf::HarmonicTable t = scarf::build_table(s.setup, sampler->draw());
+        s.host_seconds = host + std::chrono::duration<double>(clock::now() - a).count();
+        if (!s.device->upload_table(t, err)) return fail(err);
+        s.table_ready = true;
+        return run(d_field, stats, on_slab);
+    }
+    s.host_seconds = host;
+    report_stats(stats, st);
     s.error.clear();
     return 0;
 }
@@ -271,7 +355,22 @@ long field_points(const SrmConfig &cfg)
     return long(cfg.fe[0] - cfg.fs[0] + 1) * (cfg.fe[1] - cfg.fs[1] + 1) * (cfg.fe[2] - cfg.fs[2] + 1);
 }
 
+// Runs f; an exception from it is reported as the call's error instead of
+// leaving through a C entry point.
+template <class F>
+void guarded(F &&f)
+{
+    try {
+        f();
+    } catch (const std::exception &e) {
+        fail(std::string("internal error: ") + e.what());
+    } catch (...) {
+        fail("internal error");
+    }
+}
+
 }  // namespace
+
 
 extern "C" {
 
@@ -297,7 +396,7 @@ void scarf_opt_init(struct scarf_opt *var)
 
 // initialize_structured (src/scarflib.f90:97-189) behind struct_initialize
 // (src/f_binding.f90:57-113) and the option rules of src/c_binding.c:89-103
-void scarf_cart_initialize(const int nd, const int fs[], const int fe[], const real ds, const int acf,
+static void scarf_cart_initialize_body(const int nd, const int fs[], const int fe[], const real ds, const int acf,
                            const real cl[], const real sigma, struct scarf_opt *var)
 {
     State &s = state();
@@ -370,7 +469,7 @@ void scarf_cart_initialize(const int nd, const int fs[], const int fe[], const r
 
 // initialize_unstructured (src/scarflib.f90:195-296) behind unstruct_initialize
 // (src/f_binding.f90:119-175) and src/c_binding.c:116-130
-void scarf_nocart_initialize(const int nd, const int npts, const real *x, const real *y, const real *z,
+static void scarf_nocart_initialize_body(const int nd, const int npts, const real *x, const real *y, const real *z,
                              const real dh, const int acf, const real cl[], const real sigma, struct scarf_opt *var)
 {
     State &s = state();
@@ -443,7 +542,7 @@ void scarf_nocart_initialize(const int nd, const int npts, const real *x, const 
     s.error.clear();
 }
 
-void scarf_execute(const int seed, real *field, real stats[])
+static void scarf_execute_body(const int seed, real *field, real stats[])
 {
     State &s = state();
     clear_stats(stats);
@@ -451,7 +550,23 @@ void scarf_execute(const int seed, real *field, real stats[])
         fail("scarf_execute: field is NULL");
         return;
     }
-    if (prepare(seed) < 0) return;
+    if (!s.initialized) {
+        fail("execute called before initialize");
+        return;
+    }
+    if (s.cfg.method != 1) {
+        fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
+        return;
+    }
+    {  // parameter errors are reported before any device work
+        s.cfg.nharm = s.nharm;
+        std::string e;
+        if ((kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, e) : scarf::srm_setup<float>(s.cfg, s.setup, e)) != 0) {
+            fail(e);
+            return;
+        }
+    }
+    if (!ensure_device()) return;
     const long n = field_points(s.cfg);
     std::string err;
     void *d = s.device->field_buffer(size_t(n) * sizeof(real), err);
@@ -475,7 +590,7 @@ void scarf_execute(const int seed, real *field, real stats[])
                             copy_stream, e);
     };
     const bool slabs = size_t(n) * sizeof(real) > 2 * kStageBytes;
-    if (run(static_cast<real *>(d), stats, slabs ? &to_host : nullptr) != 0) return;
+    if (execute_realization(seed, static_cast<real *>(d), stats, slabs ? &to_host : nullptr) != 0) return;
     if (!slabs && !copy_to_host(field, d, size_t(n) * sizeof(real), s.device->stream(), err)) fail(err);
 }
 
@@ -491,7 +606,7 @@ void scarf_finalize(void)
 
 // Single-writer versions of write_one / write_slice (src/scarflib_common.f90:245-418,
 // :527-646): raw native-endian `real`, column-major (x fastest).
-void scarf_io_one(const int nd, const int *npts, const real *field, const char fname[], const int *nwriters)
+static void scarf_io_one_body(const int nd, const int *npts, const real *field, const char fname[], const int *nwriters)
 {
     (void)nwriters;
     if (!npts || !field || !fname || (nd != 2 && nd != 3)) {
@@ -508,7 +623,7 @@ void scarf_io_one(const int nd, const int *npts, const real *field, const char f
     if (f) std::fclose(f);
 }
 
-void scarf_io_slice(const int *npts, const char axis[], const int plane, const real *field, const char fname[])
+static void scarf_io_slice_body(const int *npts, const char axis[], const int plane, const real *field, const char fname[])
 {
     if (!npts || !axis || !field || !fname) {
         fail("scarf_io_slice: bad arguments");
@@ -566,28 +681,27 @@ int scarf_get_kernel(void)
 
 const char *scarf_last_error(void) { return state().error.c_str(); }
 
-int scarf_execute_device(const int seed, real *d_field, real stats[])
+static int scarf_execute_device_body(const int seed, real *d_field, real stats[])
 {
     clear_stats(stats);
     if (!d_field) return fail("scarf_execute_device: d_field is NULL");
-    if (prepare(seed) < 0) return -1;
-    return run(d_field, stats);
+    return execute_realization(seed, d_field, stats, nullptr);
 }
 
-int scarf_srm_draw(const int seed, double *host_seconds)
+static int scarf_srm_draw_body(const int seed, double *host_seconds)
 {
     const int n = prepare(seed);
     if (host_seconds) *host_seconds = state().host_seconds;
     return n;
 }
 
-int scarf_srm_run(real *d_field, real stats[])
+static int scarf_srm_run_body(real *d_field, real stats[])
 {
     if (!d_field) return fail("scarf_srm_run: d_field is NULL");
     return run(d_field, stats);
 }
 
-int scarf_ensemble_execute(int n, const struct scarf_member *members, real *const *d_fields, real *stats,
+static int scarf_ensemble_execute_body(int n, const struct scarf_member *members, real *const *d_fields, real *stats,
                            int kernel)
 {
     State &s = state();
@@ -723,7 +837,7 @@ int scarf_last_tile(int out[3])
     return 3;
 }
 
-int scarf_srm_run_fp64(double *d_field, double stats_out[4])
+static int scarf_srm_run_fp64_body(double *d_field, double stats_out[4])
 {
     State &s = state();
     if (!d_field) return fail("scarf_srm_run_fp64: d_field is NULL");
@@ -756,7 +870,7 @@ void *scarf_stream(void)
     return reinterpret_cast<void *>(state().device->stream());
 }
 
-int scarf_setup_values(double *out, int n)
+static int scarf_setup_values_body(double *out, int n)
 {
     State &s = state();
     if (!s.initialized) return fail("not initialized");
@@ -773,7 +887,7 @@ int scarf_setup_values(double *out, int n)
     return 0;
 }
 
-int scarf_set_option(const char *name, double value)
+static int scarf_set_option_body(const char *name, double value)
 {
     State &s = state();
     if (!s.initialized) return fail("scarf_set_option: not initialized");
@@ -782,10 +896,15 @@ int scarf_set_option(const char *name, double value)
     const std::string n(name);
     const double v = static_cast<real>(value);
     // host draw execution (never changes the harmonics drawn)
-    if (n == "draw_threads" || n == "draw_block") {
+    if (n == "draw_threads" || n == "draw_block" || n == "stream_draw" || n == "draw_chunk" || n == "stream_chunk") {
         if (value < 0) return fail("scarf_set_option: " + n + " must be >= 0");
-        if (n == "draw_threads") s.draw_threads = static_cast<int>(value);
-        else s.draw_block = static_cast<long>(value);
+        if (n == "stream_chunk" && static_cast<long>(value) % scarf::kFieldChunkHarmonics != 0)
+            return fail("scarf_set_option: stream_chunk must be a multiple of 128");
+        if (n == "stream_chunk") s.stream_chunk = static_cast<int>(value);
+        else if (n == "draw_threads") s.draw_threads = static_cast<int>(value);
+        else if (n == "draw_block") s.draw_block = static_cast<long>(value);
+        else if (n == "draw_chunk") s.draw_chunk = static_cast<int>(value);
+        else s.stream_draw = value != 0;
         s.error.clear();
         return 0;
     }
@@ -825,7 +944,7 @@ int scarf_get_config(double *out, int n)
     return m;
 }
 
-long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
+static long scarf_draw_harmonics_body(const int seed, double *harm, unsigned char *used)
 {
     State &s = state();
     if (!s.initialized) return fail("not initialized");
@@ -835,8 +954,17 @@ long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
     std::string err;
     const int rc = kDouble ? scarf::srm_setup<double>(s.cfg, su, err) : scarf::srm_setup<float>(s.cfg, su, err);
     if (rc != 0) return fail(err);
-    const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(s.cfg, su, seed, s.draw_threads, s.draw_block)
-                                          : scarf::srm_draw<float>(s.cfg, su, seed, s.draw_threads, s.draw_block);
+    // the walk cut as the streamed execute cuts it (or every draw_chunk
+    // kept harmonics): the harmonics must not depend on the cuts
+    std::unique_ptr<scarf::HarmonicSampler> w =
+        kDouble ? scarf::HarmonicSampler::create<double>(s.cfg, su, seed, s.draw_threads, s.draw_block)
+                : scarf::HarmonicSampler::create<float>(s.cfg, su, seed, s.draw_threads, s.draw_block);
+    if (s.draw_chunk > 0) {
+        for (int kept = s.draw_chunk; !w->finished(); kept += s.draw_chunk) w->draw_until(kept);
+    } else {
+        for (int kept : scarf::Device::chunk_targets(s.cfg.nharm, s.stream_chunk)) w->draw_until(kept);
+    }
+    const scarf::HarmonicDraw &d = w->draw();
     for (int l = 0; l < s.cfg.nharm; ++l) {
         harm[5 * l] = d.v1[l];
         harm[5 * l + 1] = d.v2[l];
@@ -846,6 +974,93 @@ long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
         used[l] = d.used[l];
     }
     return d.uniforms;
+}
+
+// ---- the exported entry points that allocate or start threads: an
+// exception (std::bad_alloc, std::system_error) never crosses the C ABI; it
+// becomes the entry point's error.
+void scarf_cart_initialize(const int nd, const int fs[], const int fe[], const real ds, const int acf,
+                           const real cl[], const real sigma, struct scarf_opt *var)
+{
+    guarded([&] { scarf_cart_initialize_body(nd, fs, fe, ds, acf, cl, sigma, var); });
+}
+
+void scarf_nocart_initialize(const int nd, const int npts, const real *x, const real *y, const real *z,
+                             const real dh, const int acf, const real cl[], const real sigma, struct scarf_opt *var)
+{
+    guarded([&] { scarf_nocart_initialize_body(nd, npts, x, y, z, dh, acf, cl, sigma, var); });
+}
+
+void scarf_execute(const int seed, real *field, real stats[])
+{
+    guarded([&] { scarf_execute_body(seed, field, stats); });
+}
+
+void scarf_io_one(const int nd, const int *npts, const real *field, const char fname[], const int *nwriters)
+{
+    guarded([&] { scarf_io_one_body(nd, npts, field, fname, nwriters); });
+}
+
+void scarf_io_slice(const int *npts, const char axis[], const int plane, const real *field, const char fname[])
+{
+    guarded([&] { scarf_io_slice_body(npts, axis, plane, field, fname); });
+}
+
+int scarf_execute_device(const int seed, real *d_field, real stats[])
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_execute_device_body(seed, d_field, stats); });
+    return rc;
+}
+
+int scarf_srm_draw(const int seed, double *host_seconds)
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_srm_draw_body(seed, host_seconds); });
+    return rc;
+}
+
+int scarf_srm_run(real *d_field, real stats[])
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_srm_run_body(d_field, stats); });
+    return rc;
+}
+
+int scarf_ensemble_execute(int n, const struct scarf_member *members, real *const *d_fields, real *stats,
+                           int kernel)
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_ensemble_execute_body(n, members, d_fields, stats, kernel); });
+    return rc;
+}
+
+int scarf_setup_values(double *out, int n)
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_setup_values_body(out, n); });
+    return rc;
+}
+
+int scarf_set_option(const char *name, double value)
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_set_option_body(name, value); });
+    return rc;
+}
+
+long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
+{
+    long rc = -1;
+    guarded([&] { rc = scarf_draw_harmonics_body(seed, harm, used); });
+    return rc;
+}
+
+int scarf_srm_run_fp64(double *d_field, double stats_out[4])
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_srm_run_fp64_body(d_field, stats_out); });
+    return rc;
 }
 
 }  // extern "C"

@@ -79,6 +79,29 @@ public:
     bool run(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
              double *stats_out, std::string &err, const SlabHook *on_slab = nullptr);
 
+    // Source of a realization's harmonics in draw order: feed(kept, t, last)
+    // appends to t the kept harmonics that bring their total to `kept`
+    // (fewer only when the draw has ended; last = true once every harmonic
+    // has been drawn).
+    using HarmonicFeed = std::function<bool(int kept, HarmonicTable &t, bool &last, std::string &err)>;
+    // The same realization as run() on an MFMA kernel of a structured grid,
+    // with the harmonics taken from `feed` in chunks while the device works:
+    // each chunk's record upload, operand tables and field pass are queued as
+    // soon as it is drawn, and the next chunk is drawn while they run. The
+    // field is the one run() gives on the same harmonics, bit for bit. At
+    // most nharm harmonics are kept. Returns false with *restart = true, and
+    // nothing written, when a chunk holds an amplitude that the scaled-fp16
+    // scheme can only take with a rescaled table: the caller then draws the
+    // rest and uses run().
+    // chunk > 0 (a multiple of 128) fixes the kept harmonics per chunk.
+    template <class Real>
+    bool run_streamed(const Geometry &g, int kernel, int nharm, double scaling, const Epilogue &ep, Real *d_field,
+                      double *stats_out, std::string &err, const HarmonicFeed &feed, bool *restart,
+                      const SlabHook *on_slab = nullptr, int chunk = 0);
+    // kept-harmonic totals at which run_streamed cuts the chunks of a draw of
+    // nharm harmonics (the last entry is nharm: draw to the end)
+    static std::vector<int> chunk_targets(int nharm, int chunk = 0);
+
     hipStream_t stream() const { return stream_; }
     const Timings &timings() const { return timings_; }
 
@@ -95,6 +118,12 @@ public:
 
 private:
     bool reserve(void **ptr, size_t &cap, size_t bytes, std::string &err);
+    // statistics, epilogue, field hand-over and timings after the field
+    // kernels; slab_first/slab_points name the parts whose events are slab_ev_
+    template <class Real>
+    bool finish(const Geometry &g, const Epilogue &ep, Real *d_field, int nparts, double *stats_out,
+                const std::vector<long> &slab_first, const std::vector<long> &slab_points, std::string &err,
+                const SlabHook *on_slab);
 
     int device_ = -1;
     hipStream_t stream_ = nullptr;
@@ -108,6 +137,8 @@ private:
     // harmonic record: wx, wy, wz (rad/m), phase (rad), amp; plus revolution copies
     double *d_harm_ = nullptr;
     size_t harm_cap_ = 0;
+    double *h_rec_ = nullptr;  // pinned staging of a streamed record
+    size_t h_rec_cap_ = 0;
     // per-axis tables (FP32 or FP64 revolutions, operand tables of the GEMM)
     void *d_tab_ = nullptr;
     size_t tab_cap_ = 0;

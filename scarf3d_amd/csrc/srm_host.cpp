@@ -14,6 +14,7 @@
 
 #include "quadpack.hpp"
 #include "srm_host.hpp"
+#include "worker_pool.hpp"
 #include "yarn2.hpp"
 
 namespace scarf {
@@ -202,68 +203,6 @@ struct DrawContext {
     }
 };
 
-// Fixed set of worker threads that run one job per call, all threads
-// together (the caller is worker 0).
-class WorkerPool {
-public:
-    explicit WorkerPool(int n) : n_(n)
-    {
-        for (int t = 1; t < n_; ++t) threads_.emplace_back([this, t] { loop(t); });
-    }
-    ~WorkerPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &t : threads_) t.join();
-    }
-    int size() const { return n_; }
-    void run(const std::function<void(int)> &job)
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            job_ = &job;
-            pending_ = n_ - 1;
-            ++generation_;
-        }
-        cv_.notify_all();
-        job(0);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [this] { return pending_ == 0; });
-        job_ = nullptr;
-    }
-
-private:
-    void loop(int t)
-    {
-        long seen = 0;
-        for (;;) {
-            const std::function<void(int)> *job;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
-                if (stop_) return;
-                seen = generation_;
-                job = job_;
-            }
-            (*job)(t);
-            {
-                std::lock_guard<std::mutex> lk(mu_);
-                if (--pending_ == 0) done_.notify_one();
-            }
-        }
-    }
-    int n_;
-    std::vector<std::thread> threads_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_;
-    const std::function<void(int)> *job_ = nullptr;
-    long generation_ = 0;
-    int pending_ = 0;
-    bool stop_ = false;
-};
 
 }  // namespace
 
@@ -275,6 +214,22 @@ int default_draw_threads()
     }
     const unsigned hw = std::thread::hardware_concurrency();
     return static_cast<int>(std::max(1u, std::min(hw, 16u)));
+}
+
+namespace {
+
+// A worker pool kept for the process, so that a draw does not start and join
+// its threads on every call. One draw at a time uses it; a draw that finds it
+// busy (the library is not re-entrant, but a caller may still overlap calls)
+// makes a pool of its own.
+struct SharedPool {
+    std::mutex mu;
+    std::unique_ptr<WorkerPool> pool;
+};
+SharedPool &shared_pool()
+{
+    static SharedPool *p = new SharedPool;  // never destroyed: its threads live until the process ends
+    return *p;
 }
 
 // Harmonic draw (src/scarflib_srm.f90:731-778): rejection sampling of the
@@ -289,111 +244,178 @@ int default_draw_threads()
 // only looks up the next accepted pair. The pairs consumed, and so every
 // harmonic, are those of the one-pair-at-a-time loop.
 template <class Real>
-HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed, int threads, long block_pairs)
-{
-    HarmonicDraw d;
-    const int n = cfg.nharm;
-    d.v1.assign(n, 0.0);
-    d.v2.assign(n, 0.0);
-    d.v3.assign(n, 0.0);
-    d.a.assign(n, 0.0);
-    d.b.assign(n, 0.0);
-    d.used.assign(n, 0);
+class Sampler final : public HarmonicSampler {
+public:
+    Sampler(const SrmConfig &cfg, const SrmSetup &s, int seed, int threads, long block_pairs)
+        : n_(cfg.nharm), ndim_(s.ndim)
+    {
+        d_.v1.assign(n_, 0.0);
+        d_.v2.assign(n_, 0.0);
+        d_.v3.assign(n_, 0.0);
+        d_.a.assign(n_, 0.0);
+        d_.b.assign(n_, 0.0);
+        d_.used.assign(n_, 0);
+        ctx_.spec.acf = cfg.acf;
+        ctx_.spec.ndim = s.ndim;
+        ctx_.spec.nu = static_cast<Real>(s.nu);
+        ctx_.kmin = static_cast<Real>(s.kmin);
+        ctx_.kmax = static_cast<Real>(s.kmax);
+        ctx_.kc = static_cast<Real>(s.kc);
+        ctx_.cnst = static_cast<Real>(s.cnst);
+        ctx_.seed = static_cast<unsigned long>(seed);  // set_seed: unsigned long myseed = seed
+        cl_[0] = static_cast<Real>(cfg.cl[0]);
+        cl_[1] = static_cast<Real>(cfg.cl[1]);
+        cl_[2] = static_cast<Real>(cfg.cl[2]);
+        if (threads <= 0) threads = default_draw_threads();
+        max_block_ = block_pairs > 0 ? block_pairs : (1L << 18);
+        next_size_ = std::min(max_block_, std::max(64L, 16L * n_));  // first block: a few trials per harmonic
+        if (threads > 1) {
+            SharedPool &sp = shared_pool();
+            lock_ = std::unique_lock<std::mutex>(sp.mu, std::try_to_lock);
+            if (lock_.owns_lock()) {
+                if (!sp.pool || sp.pool->size() != threads) sp.pool.reset(new WorkerPool(threads));
+                pool_ = sp.pool.get();
+            } else {
+                own_.reset(new WorkerPool(threads));
+                pool_ = own_.get();
+            }
+        }
+    }
 
-    DrawContext<Real> ctx;
-    ctx.spec.acf = cfg.acf;
-    ctx.spec.ndim = s.ndim;
-    ctx.spec.nu = static_cast<Real>(s.nu);
-    ctx.kmin = static_cast<Real>(s.kmin);
-    ctx.kmax = static_cast<Real>(s.kmax);
-    ctx.kc = static_cast<Real>(s.kc);
-    ctx.cnst = static_cast<Real>(s.cnst);
-    ctx.seed = static_cast<unsigned long>(seed);  // set_seed: unsigned long myseed = seed
-    const Real pi = kPi<Real>;
-    const Real cl0 = static_cast<Real>(cfg.cl[0]), cl1 = static_cast<Real>(cfg.cl[1]), cl2 = static_cast<Real>(cfg.cl[2]);
+    bool finished() const override { return l_ >= n_; }
+    const HarmonicDraw &draw() const override { return d_; }
+    int position() const override { return l_; }
 
-    if (threads <= 0) threads = default_draw_threads();
-    const long max_block = block_pairs > 0 ? block_pairs : (1L << 18);
-    std::unique_ptr<WorkerPool> pool;
-    if (threads > 1) pool.reset(new WorkerPool(threads));
+    int draw_until(int kept) override
+    {
+        const Real pi = kPi<Real>;
+        Real r[2];
+        while (l_ < n_ && d_.n_used < kept) {
+            const int l = l_++;
+            long q = -1;
+            while (q < 0) {
+                if (pos_ < blk_.start || pos_ >= blk_.end) fill(pos_);
+                const uint8_t *f = blk_.accept.data();
+                for (long p = pos_; p < blk_.end; ++p)
+                    if (f[p - blk_.start]) {
+                        q = p;
+                        break;
+                    }
+                if (q < 0) pos_ = blk_.end;
+            }
+            if (q + 3 >= blk_.end) fill(q);  // the trial and the (up to) three pairs after it
+            pair(q, r);
+            const Real u = ctx_.radial(r[0]);
+            pair(q + 1, r);
+            if (u > ctx_.kc) {  // dropped, still counted by the scaling
+                pos_ = q + 2;
+                d_.uniforms = 2 * pos_;
+                continue;
+            }
+            const Real phi = r[0] * Real(2) * pi;
+            Real v1, v2, v3;
+            if (ndim_ == 3) {
+                const Real theta = std::acos(Real(1) - Real(2) * r[1]);
+                v1 = u * std::sin(phi) * std::sin(theta) / cl_[0];
+                v2 = u * std::cos(phi) * std::sin(theta) / cl_[1];
+                v3 = u * std::cos(theta) / cl_[2];
+            } else {
+                v1 = u * std::cos(phi) / cl_[0];
+                v2 = u * std::sin(phi) / cl_[1];
+                v3 = 0;
+            }
+            pair(q + 2, r);
+            const Real a = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
+            pair(q + 3, r);
+            const Real b = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
+            pos_ = q + 4;
+            d_.v1[l] = v1;
+            d_.v2[l] = v2;
+            d_.v3[l] = v3;
+            d_.a[l] = a;
+            d_.b[l] = b;
+            d_.used[l] = 1;
+            ++d_.n_used;
+            d_.uniforms = 2 * pos_;
+        }
+        return d_.n_used;
+    }
 
-    PairBlock<Real> blk;
-    long next_size = std::min(max_block, std::max(64L, 16L * n));  // first block: a few trials per harmonic
+    HarmonicDraw take() { return std::move(d_); }
+
+private:
     // make pairs [first, first + >= 4) resident
-    auto fill = [&](long first) {
-        const long count = std::max(next_size, 4L);
-        next_size = std::min(max_block, 2 * count);
-        blk.start = first;
-        blk.end = first + count;
-        blk.raw.resize(2 * static_cast<size_t>(count));
-        blk.accept.resize(static_cast<size_t>(count));
-        const int nt = (pool && count >= 4096) ? pool->size() : 1;
+    void fill(long first)
+    {
+        const long count = std::max(next_size_, 4L);
+        next_size_ = std::min(max_block_, 2 * count);
+        blk_.start = first;
+        blk_.end = first + count;
+        blk_.raw.resize(2 * static_cast<size_t>(count));
+        blk_.accept.resize(static_cast<size_t>(count));
+        const int nt = (pool_ && count >= 4096) ? pool_->size() : 1;
         if (nt == 1) {
-            ctx.fill_range(blk, first, first + count);
+            ctx_.fill_range(blk_, first, first + count);
             return;
         }
         const long chunk = (count + nt - 1) / nt;
         const std::function<void(int)> job = [&](int t) {
             const long lo = first + t * chunk, hi = std::min(first + count, lo + chunk);
-            if (lo < hi) ctx.fill_range(blk, lo, hi);
+            if (lo < hi) ctx_.fill_range(blk_, lo, hi);
         };
-        pool->run(job);
-    };
-    auto pair = [&](long p, Real r[2]) {
-        const size_t i = static_cast<size_t>(p - blk.start);
-        r[0] = DrawContext<Real>::uniform(blk.raw[2 * i]);
-        r[1] = DrawContext<Real>::uniform(blk.raw[2 * i + 1]);
-    };
-
-    long pos = 0;  // next pair the loop would read
-    Real r[2];
-    for (int l = 0; l < n; ++l) {
-        long q = -1;
-        while (q < 0) {
-            if (pos < blk.start || pos >= blk.end) fill(pos);
-            const uint8_t *f = blk.accept.data();
-            for (long p = pos; p < blk.end; ++p)
-                if (f[p - blk.start]) {
-                    q = p;
-                    break;
-                }
-            if (q < 0) pos = blk.end;
-        }
-        if (q + 3 >= blk.end) fill(q);  // the trial and the (up to) three pairs after it
-        pair(q, r);
-        const Real u = ctx.radial(r[0]);
-        pair(q + 1, r);
-        if (u > ctx.kc) {  // dropped, still counted by the scaling
-            pos = q + 2;
-            continue;
-        }
-        const Real phi = r[0] * Real(2) * pi;
-        Real v1, v2, v3;
-        if (s.ndim == 3) {
-            const Real theta = std::acos(Real(1) - Real(2) * r[1]);
-            v1 = u * std::sin(phi) * std::sin(theta) / cl0;
-            v2 = u * std::cos(phi) * std::sin(theta) / cl1;
-            v3 = u * std::cos(theta) / cl2;
-        } else {
-            v1 = u * std::cos(phi) / cl0;
-            v2 = u * std::sin(phi) / cl1;
-            v3 = 0;
-        }
-        pair(q + 2, r);
-        const Real a = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
-        pair(q + 3, r);
-        const Real b = std::sqrt(-(Real(2) * std::log(r[0]))) * std::cos(Real(2) * pi * r[1]);
-        pos = q + 4;
-        d.v1[l] = v1;
-        d.v2[l] = v2;
-        d.v3[l] = v3;
-        d.a[l] = a;
-        d.b[l] = b;
-        d.used[l] = 1;
-        ++d.n_used;
+        pool_->run(job);
     }
-    d.uniforms = 2 * pos;
-    return d;
+    void pair(long p, Real r[2]) const
+    {
+        const size_t i = static_cast<size_t>(p - blk_.start);
+        r[0] = DrawContext<Real>::uniform(blk_.raw[2 * i]);
+        r[1] = DrawContext<Real>::uniform(blk_.raw[2 * i + 1]);
+    }
+
+    const int n_, ndim_;
+    Real cl_[3];
+    DrawContext<Real> ctx_;
+    HarmonicDraw d_;
+    PairBlock<Real> blk_;
+    long max_block_ = 0, next_size_ = 0;
+    long pos_ = 0;  // next pair the loop would read
+    int l_ = 0;     // next harmonic index
+    std::unique_lock<std::mutex> lock_;
+    std::unique_ptr<WorkerPool> own_;
+    WorkerPool *pool_ = nullptr;
+};
+
+}  // namespace
+
+template <class Real>
+std::unique_ptr<HarmonicSampler> HarmonicSampler::create(const SrmConfig &cfg, const SrmSetup &setup, int seed,
+                                                         int threads, long block_pairs)
+{
+    return std::unique_ptr<HarmonicSampler>(new Sampler<Real>(cfg, setup, seed, threads, block_pairs));
+}
+
+template <class Real>
+HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &s, int seed, int threads, long block_pairs)
+{
+    Sampler<Real> w(cfg, s, seed, threads, block_pairs);
+    w.draw_until(cfg.nharm);
+    return w.take();
+}
+
+void append_table(const SrmSetup &s, const HarmonicDraw &d, int l0, int l1, HarmonicTable &t)
+{
+    const double *m = s.matrix;
+    for (int l = l0; l < l1; ++l) {
+        if (!d.used[l]) continue;
+        const double v[3] = {d.v1[l], d.v2[l], d.v3[l]};
+        // w = M^T v: theta = sum_r v_r ((M p)_r + bar_r) = (M^T v) . p + v . bar
+        t.wx.push_back(v[0] * m[0] + v[1] * m[1] + v[2] * m[2]);
+        t.wy.push_back(v[0] * m[3] + v[1] * m[4] + v[2] * m[5]);
+        t.wz.push_back(v[0] * m[6] + v[1] * m[7] + v[2] * m[8]);
+        const double vbar = v[0] * s.bar[0] + v[1] * s.bar[1] + v[2] * s.bar[2];
+        t.phase.push_back(vbar - std::atan2(d.a[l], d.b[l]));
+        t.amp.push_back(std::hypot(d.a[l], d.b[l]));
+    }
 }
 
 HarmonicTable build_table(const SrmSetup &s, const HarmonicDraw &d)
@@ -405,18 +427,7 @@ HarmonicTable build_table(const SrmSetup &s, const HarmonicDraw &d)
     t.wz.reserve(n);
     t.phase.reserve(n);
     t.amp.reserve(n);
-    const double *m = s.matrix;
-    for (size_t l = 0; l < d.used.size(); ++l) {
-        if (!d.used[l]) continue;
-        const double v[3] = {d.v1[l], d.v2[l], d.v3[l]};
-        // w = M^T v: theta = sum_r v_r ((M p)_r + bar_r) = (M^T v) . p + v . bar
-        t.wx.push_back(v[0] * m[0] + v[1] * m[1] + v[2] * m[2]);
-        t.wy.push_back(v[0] * m[3] + v[1] * m[4] + v[2] * m[5]);
-        t.wz.push_back(v[0] * m[6] + v[1] * m[7] + v[2] * m[8]);
-        const double vbar = v[0] * s.bar[0] + v[1] * s.bar[1] + v[2] * s.bar[2];
-        t.phase.push_back(vbar - std::atan2(d.a[l], d.b[l]));
-        t.amp.push_back(std::hypot(d.a[l], d.b[l]));
-    }
+    append_table(s, d, 0, static_cast<int>(d.used.size()), t);
     return t;
 }
 
@@ -424,5 +435,9 @@ template int srm_setup<float>(const SrmConfig &, SrmSetup &, std::string &);
 template int srm_setup<double>(const SrmConfig &, SrmSetup &, std::string &);
 template HarmonicDraw srm_draw<float>(const SrmConfig &, const SrmSetup &, int, int, long);
 template HarmonicDraw srm_draw<double>(const SrmConfig &, const SrmSetup &, int, int, long);
+template std::unique_ptr<HarmonicSampler> HarmonicSampler::create<float>(const SrmConfig &, const SrmSetup &, int,
+                                                                         int, long);
+template std::unique_ptr<HarmonicSampler> HarmonicSampler::create<double>(const SrmConfig &, const SrmSetup &, int,
+                                                                          int, long);
 
 }  // namespace scarf

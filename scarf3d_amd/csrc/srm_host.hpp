@@ -4,6 +4,7 @@
 // the harmonic table the kernels consume.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -67,9 +68,29 @@ struct HarmonicTable {
 template <class Real>
 int srm_setup(const SrmConfig &cfg, SrmSetup &out, std::string &err);
 
+// The harmonic draw of one seed as a resumable walk: draw_until(m) runs the
+// reference's draw loop until m harmonics have been kept (or every harmonic
+// has been drawn), so that a caller can hand the first harmonics to the
+// device while the rest are still drawn. Where the walk is cut never
+// changes the harmonics: srm_draw is one draw_until to the end.
 // threads <= 0: default_draw_threads(); block_pairs <= 0: 2^18 pairs per
 // block at most (smaller blocks only change how the work is cut, never the
-// harmonics)
+// harmonics).
+class HarmonicSampler {
+public:
+    template <class Real>
+    static std::unique_ptr<HarmonicSampler> create(const SrmConfig &cfg, const SrmSetup &setup, int seed,
+                                                   int threads = 0, long block_pairs = 0);
+    virtual ~HarmonicSampler() = default;
+    // returns the number of kept harmonics so far
+    virtual int draw_until(int kept) = 0;
+    // every harmonic 1..nharm has been drawn
+    virtual bool finished() const = 0;
+    // harmonics drawn so far: v/a/b/used for indices [0, position())
+    virtual const HarmonicDraw &draw() const = 0;
+    virtual int position() const = 0;
+};
+
 template <class Real>
 HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &setup, int seed, int threads = 0, long block_pairs = 0);
 
@@ -77,5 +98,7 @@ HarmonicDraw srm_draw(const SrmConfig &cfg, const SrmSetup &setup, int seed, int
 int default_draw_threads();
 
 HarmonicTable build_table(const SrmSetup &setup, const HarmonicDraw &draw);
+// the kept harmonics among indices [l0, l1) appended to t
+void append_table(const SrmSetup &setup, const HarmonicDraw &draw, int l0, int l1, HarmonicTable &t);
 
 }  // namespace scarf

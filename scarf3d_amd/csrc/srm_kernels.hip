@@ -465,6 +465,7 @@ Device::~Device()
     for (void *b : bufs)
         if (b) hipFree(b);
     if (h_stat_) hipHostFree(h_stat_);
+    if (h_rec_) hipHostFree(h_rec_);
     for (auto &e : ev_)
         if (e) hipEventDestroy(e);
     for (auto &e : slab_ev_)
@@ -680,6 +681,15 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
                                                                     d_harm_, np_, scaling, d_field, d_part_);
         HIP_TRY(hipGetLastError());
     }
+    return finish<Real>(g, ep, d_field, nparts, stats_out, slab_first, slab_points, err, on_slab);
+}
+
+template <class Real>
+bool Device::finish(const Geometry &g, const Epilogue &ep, Real *d_field, int nparts, double *stats_out,
+                    const std::vector<long> &slab_first, const std::vector<long> &slab_points, std::string &err,
+                    const SlabHook *on_slab)
+{
+    const long npts = g.points();
     HIP_TRY(hipEventRecord(ev_[2], stream_));
     stats_finalize<Real><<<1, 256, 0, stream_>>>(d_part_, nparts, ep.sigma, d_stat_);
     HIP_TRY(hipGetLastError());
@@ -728,11 +738,160 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
     return true;
 }
 
+std::vector<int> Device::chunk_targets(int nharm, int chunk)
+{
+    std::vector<int> t;
+    if (chunk > 0) {
+        for (long kept = chunk; kept < nharm; kept += chunk) t.push_back(static_cast<int>(kept));
+        t.push_back(nharm);
+        return t;
+    }
+    // The first chunk is small, so that the device starts after a short part
+    // of the draw; each later one doubles, since the draw runs several times
+    // faster than the device consumes it. A tail shorter than the chunk
+    // before it joins that chunk. Every cut but the last is a multiple of the
+    // promotion period of the field kernel.
+    const int q = kFieldChunkHarmonics;
+    long c = std::max<long>(8 * q, round_up(nharm / 16, q));
+    long kept = 0;
+    while (kept + 2 * c < nharm) {  // what is left after this chunk is at least a chunk
+        kept += c;
+        t.push_back(static_cast<int>(kept));
+        c *= 2;
+    }
+    t.push_back(nharm);
+    return t;
+}
+
+template <class Real>
+bool Device::run_streamed(const Geometry &g, int kernel, int nharm, double scaling, const Epilogue &ep,
+                          Real *d_field, double *stats_out, std::string &err, const HarmonicFeed &feed,
+                          bool *restart, const SlabHook *on_slab, int chunk)
+{
+    if (restart) *restart = false;
+    if (chunk < 0 || chunk % kFieldChunkHarmonics != 0) {
+        err = "the streamed chunk must be a multiple of 128 harmonics";
+        return false;
+    }
+    HIP_TRY(hipSetDevice(device_));
+    const bool mfma = kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 || kernel == kKernelMfmaBf16x2;
+    if (!g.structured || !mfma || nharm < 1) {
+        err = "streamed realizations run on the MFMA kernels of structured grids";
+        return false;
+    }
+    TileChoice tile;
+    if (!TileTuner::forced(tile) && TileTuner::enabled() &&
+        !tuner().choose<Real>(g, kernel, d_field, stream_, tile, err))
+        return false;
+    // record and tables sized for every harmonic plus one spare padding block:
+    // a draw whose last harmonics are all dropped ends with an empty chunk,
+    // which then runs one block of zero-amplitude harmonics (each adds 0)
+    const int cap = static_cast<int>(round_up(nharm, kHarmPad)) + kHarmPad;
+    MfmaPlan plan;
+    if (!mfma_plan(g, cap, kernel, plan, err, tile.nbx, tile.tk)) return false;
+    last_nbx_ = plan.nbx;
+    last_tk_ = plan.tk;
+    plan.zscale = 1.0;  // a chunk that would need another scale restarts on run()
+    const size_t rec_bytes = size_t(5) * cap * sizeof(double);
+    void *p = d_harm_;
+    if (!reserve(&p, harm_cap_, rec_bytes, err)) return false;
+    d_harm_ = static_cast<double *>(p);
+    nharm_ = 0;  // the streamed record is not a resident table for run()
+    if (h_rec_cap_ < rec_bytes) {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        if (h_rec_) HIP_TRY(hipHostFree(h_rec_));
+        h_rec_ = nullptr;
+        h_rec_cap_ = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&h_rec_), rec_bytes, hipHostMallocDefault));
+        h_rec_cap_ = rec_bytes;
+    }
+    p = d_tab_;
+    if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;
+    d_tab_ = p;
+    const int nparts = plan.nparts;
+    p = d_part_;
+    if (!reserve(&p, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
+    d_part_ = static_cast<double *>(p);
+
+    timings_ = Timings{};
+    HIP_TRY(hipEventRecord(ev_[0], stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));  // the staging buffer is free
+    const std::vector<int> targets = chunk_targets(nharm, chunk);
+    std::vector<long> slab_first, slab_points;
+    HarmonicTable t;
+    int kept = 0;  // harmonics queued so far (padding included in the last chunk)
+    bool last = false;
+    for (size_t c = 0; !last; ++c) {
+        t = HarmonicTable{};
+        // the host draws this chunk while the device runs the chunks before it
+        if (!feed(c < targets.size() ? targets[c] : nharm, t, last, err)) return false;
+        const int n = t.count();
+        double amp_max = 0;
+        for (int l = 0; l < n; ++l) amp_max = std::max(amp_max, t.amp[l]);
+        if (mfma_zscale(kernel, amp_max) != 1.0) {
+            HIP_TRY(hipStreamSynchronize(stream_));
+            if (restart) *restart = true;
+            err = "amplitude beyond the scaled-fp16 range: the realization needs a rescaled table";
+            return false;
+        }
+        if (kept + n > cap - kHarmPad || (!last && n % kFieldChunkHarmonics != 0)) {
+            err = "harmonic feed returned more harmonics than asked for";
+            return false;
+        }
+        int nh = last ? static_cast<int>(round_up(kept + n, kHarmPad)) - kept : n;
+        if (nh == 0) nh = kHarmPad;
+        // the chunk's part of each record field, zero-padded, staged in
+        // pinned memory (each chunk has its own range: nothing is rewritten
+        // while its copy may still run)
+        const double *src[5] = {t.wx.data(), t.wy.data(), t.wz.data(), t.phase.data(), t.amp.data()};
+        for (int f = 0; f < 5; ++f) {
+            double *dst = h_rec_ + size_t(f) * cap + kept;
+            std::copy(src[f], src[f] + n, dst);
+            std::fill(dst + n, dst + nh, 0.0);
+            HIP_TRY(hipMemcpyAsync(d_harm_ + size_t(f) * cap + kept, dst, size_t(nh) * sizeof(double),
+                                   hipMemcpyHostToDevice, stream_));
+        }
+        if (!mfma_tables(plan, g, d_harm_, cap, d_tab_, stream_, err, kept, nh)) return false;
+        if (c == 0) HIP_TRY(hipEventRecord(ev_[1], stream_));
+        const int s0 = kept / 16, ns = nh / 16;
+        const int flags = (kept > 0 ? kFieldCarryIn : 0) | (last ? kFieldFinal : 0);
+        if (last && on_slab && !ep.active() && plan.tiles_k > 1) {
+            // the final chunk in z slabs, each handed over as soon as it is final
+            const long row = long(plan.tiles_x) * plan.tiles_j;
+            const int nslab = std::min(kSlabs, plan.tiles_k);
+            const int per = (plan.tiles_k + nslab - 1) / nslab;
+            const long plane = long(g.nx) * g.ny;
+            for (int k0 = 0; k0 < plan.tiles_k; k0 += per) {
+                const int k1 = std::min(k0 + per, plan.tiles_k);
+                const int sl = int(slab_first.size());
+                if (!slab_ev_[sl]) HIP_TRY(hipEventCreateWithFlags(&slab_ev_[sl], hipEventDisableTiming));
+                if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err, (k1 - k0) * row, k0 * row,
+                                s0, ns, flags))
+                    return false;
+                HIP_TRY(hipEventRecord(slab_ev_[sl], stream_));
+                const long z0 = long(k0) * plan.tk, z1 = std::min<long>(long(k1) * plan.tk, g.nz);
+                slab_first.push_back(z0 * plane);
+                slab_points.push_back((z1 - z0) * plane);
+            }
+        } else if (!mfma_field(plan, g, d_tab_, scaling, d_field, d_part_, stream_, err, 0, 0, s0, ns, flags)) {
+            return false;
+        }
+        kept += nh;
+    }
+    return finish<Real>(g, ep, d_field, nparts, stats_out, slab_first, slab_points, err, on_slab);
+}
+
 template bool Device::upload_points<float>(const float *, const float *, const float *, long, std::string &);
 template bool Device::upload_points<double>(const double *, const double *, const double *, long, std::string &);
 template bool Device::run<float>(const Geometry &, int, double, const Epilogue &, float *, double *, std::string &,
                                  const SlabHook *);
 template bool Device::run<double>(const Geometry &, int, double, const Epilogue &, double *, double *,
                                   std::string &, const SlabHook *);
+template bool Device::run_streamed<float>(const Geometry &, int, int, double, const Epilogue &, float *, double *,
+                                          std::string &, const HarmonicFeed &, bool *, const SlabHook *,
+                                          int);
+template bool Device::run_streamed<double>(const Geometry &, int, int, double, const Epilogue &, double *, double *,
+                                           std::string &, const HarmonicFeed &, bool *, const SlabHook *,
+                                          int);
 
 }  // namespace scarf

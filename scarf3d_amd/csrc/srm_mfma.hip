@@ -46,6 +46,7 @@ constexpr int kTileX = 128;     // x-points per block
 constexpr int kTileC = 256;     // grid columns per block
 constexpr int kThreads = 512;
 constexpr int kPromote = 8;     // stages between promotions
+static_assert(kFieldChunkHarmonics == kPromote * kStageH, "chunks start at promotion boundaries");
 constexpr double kInvTwoPi = 0.15915494309189533577;
 
 enum Scheme { kBf16x3 = 0, kBf16x2 = 1, kF16x2s = 2 };
@@ -199,10 +200,11 @@ __device__ inline uint32_t half_bits(typename SchemeOf<S>::elt v)
 }
 // one realization's table parameters (nx_pad, and the table pointers, from its plan)
 struct TableArgs {
-    const double *harm;
+    const double *harm;  // SoA record, np_ values per field
     void *xt;
     float2 *yt, *zt;
     int np_, nx, nx_pad, ny, nz, ox, oy, oz;
+    int h0, nh;          // harmonics [h0, h0 + nh) (multiples of the stage)
     double ds, zscale;
 };
 
@@ -216,11 +218,12 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
     E *__restrict__ xt = static_cast<E *>(a.xt);
     const int np_ = a.np_, nx = a.nx, nx_pad = a.nx_pad, ny = a.ny, nz = a.nz;
     const double ds = a.ds;
-    const long nxi = long(np_ / kStageH) * nx_pad, nyi = long(np_) * ny;
-    const long total = nxi + nyi + long(np_) * nz;
+    const int h0 = a.h0, nh = a.nh;
+    const long nxi = long(nh / kStageH) * nx_pad, nyi = long(nh) * ny;
+    const long total = nxi + nyi + long(nh) * nz;
     for (long e = first; e < total; e += step) {
         if (e < nxi) {
-            const int stage = int(e / nx_pad), r = int(e % nx_pad);
+            const int stage = h0 / kStageH + int(e / nx_pad), r = int(e % nx_pad);
             uint32_t words[P][kStageH];  // harmonic h: (cos, -sin) parts as one little-endian word
 #pragma unroll
             for (int h = 0; h < kStageH; ++h) {
@@ -248,14 +251,14 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
                     row[q] = uint4{words[p][4 * q], words[p][4 * q + 1], words[p][4 * q + 2], words[p][4 * q + 3]};
             }
         } else if (e < nxi + nyi) {
-            const int n = int((e - nxi) / ny), j = int((e - nxi) % ny);
+            const int n = h0 + int((e - nxi) / ny), j = int((e - nxi) % ny);
             double t = (harm[np_ + n] * (double(j + a.oy) * ds)) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
             a.yt[size_t(n) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
         } else {
-            const int n = int((e - nxi - nyi) / nz), k = int((e - nxi - nyi) % nz);
+            const int n = h0 + int((e - nxi - nyi) / nz), k = int((e - nxi - nyi) % nz);
             double t = (harm[2 * np_ + n] * (double(k + a.oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
             t -= rint(t);
             double sd, cd;
@@ -319,8 +322,17 @@ __device__ inline void lds_dma4(const char *base, uint32_t off, void *lds_wave_b
                                      (__attribute__((address_space(3))) void *)lds_wave_base, 4, 0, 0);
 }
 
+// Flags of a field launch over a range of stages. A realization whose
+// harmonics arrive in chunks runs one launch per chunk: every launch but the
+// last stores its tile's unscaled FP32 running sums into the field, and
+// every launch but the first starts from them. Chunks start at multiples of
+// the promotion period, so each value sums the same products in the same
+// order as one launch over all stages.
+constexpr int kCarryIn = kFieldCarryIn;  // start from the running sums stored in the field
+constexpr int kFinal = kFieldFinal;      // scale, round, store and take the statistics
+
 // One output tile: 256 grid columns (tj values of j by tk values of k,
-// starting at j0, k0) by TX = 32*NBX x-points starting at x0, over all
+// starting at j0, k0) by TX = 32*NBX x-points starting at x0, over
 // `stages` stages of 16 harmonics. Every thread of the block calls this with
 // the same arguments.
 //
@@ -336,7 +348,7 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
                                           int ny, int nz, int nx_pad, int stages, int tj_shift, int tk, int x0,
                                           int j0, int k0, double scaling, Out *__restrict__ field,
                                           double *__restrict__ part_out, StageLds<SchemeOf<S>::parts> &lds0,
-                                          StageLds<SchemeOf<S>::parts> &lds1)
+                                          StageLds<SchemeOf<S>::parts> &lds1, int flags = kFinal)
 {
     using Sch = SchemeOf<S>;
     using E = typename Sch::elt;
@@ -394,6 +406,25 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
 #pragma unroll
         for (int a = 0; a < A; ++a) acc[a][bx] = f32x16{};
         main_acc[bx] = f32x16{};
+    }
+    // output value r of x block bx: grid position of this lane's entry
+    auto out_index = [&](int bx, int r, bool &inside) __attribute__((always_inline)) {
+        const int x = x0 + bx * 32 + (lane & 31);
+        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int cl = wave * 32 + row;
+        const int j = j0 + (cl & tj_mask), k = k0 + (cl >> tj_shift);
+        inside = x < nx && j < ny && k < nz;
+        return size_t(x) + size_t(nx) * (size_t(j) + size_t(ny) * k);
+    };
+    if (flags & kCarryIn) {
+#pragma unroll
+        for (int bx = 0; bx < NBX; ++bx)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                bool inside;
+                const size_t idx = out_index(bx, r, inside);
+                main_acc[bx][r] = inside ? static_cast<float>(field[idx]) : 0.f;
+            }
     }
 
     __syncthreads();  // the LDS buffers may still be read by the previous tile
@@ -549,6 +580,17 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
         stage(s + 1, lds1, lds0);
     }
 
+    if (!(flags & kFinal)) {  // running sums for the next chunk
+#pragma unroll
+        for (int bx = 0; bx < NBX; ++bx)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                bool inside;
+                const size_t idx = out_index(bx, r, inside);
+                if (inside) field[idx] = static_cast<Out>(main_acc[bx][r]);
+            }
+        return;
+    }
     // epilogue: scale, round to the field type, store, statistics
     double vals[16 * NBX];
     bool ok[16 * NBX];
@@ -570,13 +612,16 @@ __device__ __forceinline__ void gemm_tile(const typename SchemeOf<S>::elt *__res
     block_moments(local_moments(vals, ok), part_out);
 }
 
-template <int S, int NBX, class Out>
+// CHUNK = false: one launch over every stage (flags is kFinal); true: one
+// chunk of a realization (flags as passed), a separate instantiation so that
+// the single-launch kernel keeps its registers.
+template <int S, int NBX, class Out, bool CHUNK>
 __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf<S>::elt *__restrict__ xt,
                                                          const float2 *__restrict__ yt,
                                                          const float2 *__restrict__ zt, int nx, int ny, int nz,
                                                          int nx_pad, int stages, int tiles_x, int tj_shift, int tk,
                                                          int tiles_j, double scaling, Out *__restrict__ field,
-                                                         double *__restrict__ part, long first)
+                                                         double *__restrict__ part, long first, int flags)
 {
     __shared__ StageLds<SchemeOf<S>::parts> lds0, lds1;
     const long b = first + blockIdx.x;
@@ -585,7 +630,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_grid(const typename SchemeOf
     const int j0 = int(ct % tiles_j) << tj_shift;
     const int k0 = int(ct / tiles_j) * tk;
     gemm_tile<S, NBX, Out>(xt, yt, zt, nx, ny, nz, nx_pad, stages, tj_shift, tk, x0, j0, k0, scaling, field,
-                           part + 3 * b, lds0, lds1);
+                           part + 3 * b, lds0, lds1, CHUNK ? flags : kFinal);
 }
 
 // Ensemble: one persistent launch over the tiles of many realizations.
@@ -855,16 +900,26 @@ TableArgs table_args(const MfmaPlan &plan, const Geometry &g, const double *d_ha
     a.oz = g.oz;
     a.ds = g.ds;
     a.zscale = plan.zscale;
+    a.h0 = 0;
+    a.nh = np_;
     return a;
 }
 
 }  // namespace
 
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
-                 hipStream_t stream, std::string &err)
+                 hipStream_t stream, std::string &err, int h0, int nh)
 {
-    const TableArgs a = table_args(plan, g, d_harm, np_, d_tab);
-    const long total = long(np_ / kStageH) * plan.nx_pad + long(np_) * (g.ny + g.nz);  // work items
+    TableArgs a = table_args(plan, g, d_harm, np_, d_tab);
+    if (nh < 0) nh = np_ - h0;
+    if (h0 < 0 || h0 % kStageH != 0 || nh % kStageH != 0 || h0 + nh > np_ || h0 + nh > plan.np_) {
+        err = "mfma_tables: harmonic range outside the table";
+        return false;
+    }
+    if (nh == 0) return true;
+    a.h0 = h0;
+    a.nh = nh;
+    const long total = long(nh / kStageH) * plan.nx_pad + long(nh) * (g.ny + g.nz);  // work items
     const int blocks = int(std::min<long>((total + 255) / 256, 65536));
     switch (scheme_of(plan.kernel)) {
     case kBf16x3: mfma_operand_tables<kBf16x3><<<blocks, 256, 0, stream>>>(a); break;
@@ -883,19 +938,29 @@ namespace {
 
 template <int S, class Real>
 void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, double scaling, Real *d_field,
-                 double *d_part, hipStream_t stream, long blocks, long first)
+                 double *d_part, hipStream_t stream, long blocks, long first, int s0, int nstages, int flags)
 {
     using E = typename SchemeOf<S>::elt;
-    const E *xt = reinterpret_cast<const E *>(base);
-    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes);
-    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes);
+    // operand tables from stage s0 on: X is stage-major, Y and Z harmonic-major
+    const E *xt = reinterpret_cast<const E *>(base) + size_t(s0) * SchemeOf<S>::parts * plan.nx_pad * kStageK;
+    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes) + size_t(s0) * kStageH * g.ny;
+    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes) +
+                       size_t(s0) * kStageH * g.nz;
     const dim3 grid(static_cast<unsigned>(blocks));
-#define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, plan.stages, plan.tiles_x, plan.tj_shift, plan.tk, \
-                        plan.tiles_j, scaling, d_field, d_part, first
-    switch (plan.nbx) {
-    case 2: mfma_grid<S, 2, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
-    case 3: mfma_grid<S, 3, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
-    default: mfma_grid<S, 4, Real><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+#define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, nstages, plan.tiles_x, plan.tj_shift, plan.tk, \
+                        plan.tiles_j, scaling, d_field, d_part, first, flags
+    if (flags == kFinal) {
+        switch (plan.nbx) {
+        case 2: mfma_grid<S, 2, Real, false><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        case 3: mfma_grid<S, 3, Real, false><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        default: mfma_grid<S, 4, Real, false><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        }
+    } else {
+        switch (plan.nbx) {
+        case 2: mfma_grid<S, 2, Real, true><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        case 3: mfma_grid<S, 3, Real, true><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        default: mfma_grid<S, 4, Real, true><<<grid, kThreads, 0, stream>>>(SCARF_GRID_ARGS); break;
+        }
     }
 #undef SCARF_GRID_ARGS
 }
@@ -904,7 +969,8 @@ void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, doub
 
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err, long max_blocks, long first_block)
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks, long first_block, int s0,
+                int nstages, int flags)
 {
     const char *base = static_cast<const char *>(d_tab);
     scaling /= plan.zscale;  // exact: a power of two
@@ -912,13 +978,27 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
         err = "mfma_field: first tile out of range";
         return false;
     }
+    if (nstages < 0) nstages = plan.stages - s0;
+    // a chunk starts at a promotion boundary and holds at least one stage
+    if (s0 < 0 || nstages < 1 || s0 + nstages > plan.stages || s0 % kPromote != 0 ||
+        (flags & ~(kFieldCarryIn | kFieldFinal)) != 0 || ((flags & kFieldCarryIn) == 0) != (s0 == 0) ||
+        (((flags & kFieldFinal) == 0) && nstages % kPromote != 0)) {
+        err = "mfma_field: stage range or flags do not fit the table";
+        return false;
+    }
     const long left = plan.blocks - first_block;
     const long blocks = max_blocks > 0 ? std::min(left, max_blocks) : left;
     const long f = first_block;
     switch (scheme_of(plan.kernel)) {
-    case kBf16x3: launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
-    case kBf16x2: launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
-    default: launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f); break;
+    case kBf16x3:
+        launch_grid<kBf16x3, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f, s0, nstages, flags);
+        break;
+    case kBf16x2:
+        launch_grid<kBf16x2, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f, s0, nstages, flags);
+        break;
+    default:
+        launch_grid<kF16x2s, Real>(plan, g, base, scaling, d_field, d_part, stream, blocks, f, s0, nstages, flags);
+        break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -929,7 +1009,7 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
 }
 
 template bool mfma_field<float>(const MfmaPlan &, const Geometry &, const void *, double, float *, double *,
-                                hipStream_t, std::string &, long, long);
+                                 hipStream_t, std::string &, long, long, int, int, int);
 
 // ---------------------------------------------------------------------------
 Ensemble::~Ensemble() { release(); }
@@ -1448,6 +1528,6 @@ template bool TileTuner::choose<float>(const Geometry &, int, float *, hipStream
 template bool TileTuner::choose<double>(const Geometry &, int, double *, hipStream_t, TileChoice &, std::string &);
 
 template bool mfma_field<double>(const MfmaPlan &, const Geometry &, const void *, double, double *, double *,
-                                 hipStream_t, std::string &, long, long);
+                                 hipStream_t, std::string &, long, long, int, int, int);
 
 }  // namespace scarf

@@ -53,14 +53,32 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
 // The same with mixed x tiles: 128-point tiles and at most two tails of 64
 // or 96 points, which cover any multiple of 32 points without padding
 bool mfma_plan_mixed(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int tk = 0);
+// Operand tables of harmonics [h0, h0 + nh) of a record of np_ harmonics per
+// field (all of them when nh < 0); h0 and nh are multiples of 16.
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
-                 hipStream_t stream, std::string &err);
+                 hipStream_t stream, std::string &err, int h0 = 0, int nh = -1);
+
+// Passes of a field launch over a range of stages (16 harmonics each): a
+// realization whose harmonics arrive in chunks runs one launch per chunk.
+// Every launch but the first carries in the unscaled FP32 running sums
+// stored in the field by the one before; only the last (kFieldFinal)
+// scales, rounds and stores the field and writes the statistics partials.
+// Chunks start at multiples of 8 stages (the promotion period) and all but
+// the last hold a multiple of 8 stages, so each value sums the same products
+// in the same order as one launch over every stage: the field is the same
+// bit for bit.
+constexpr int kFieldCarryIn = 1;
+constexpr int kFieldFinal = 2;
+constexpr int kFieldChunkHarmonics = 128;  // chunk starts are multiples of this
+
 // Launches tiles first_block .. first_block + max_blocks - 1 (all from
-// first_block on when max_blocks is 0). Fewer than all tiles leave the field
+// first_block on when max_blocks is 0) over stages [s0, s0 + nstages) (to
+// the end when nstages < 0). Fewer than all tiles leave the field
 // incomplete: the tile tuner's timing runs, or a field computed in parts.
 template <class Real>
 bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, double scaling, Real *d_field,
-                double *d_part, hipStream_t stream, std::string &err, long max_blocks = 0, long first_block = 0);
+                double *d_part, hipStream_t stream, std::string &err, long max_blocks = 0, long first_block = 0,
+                int s0 = 0, int nstages = -1, int flags = kFieldFinal);
 
 // One realization of an ensemble, prepared on the host.
 struct EnsembleMember {

@@ -205,18 +205,5 @@ class RefQuadpack:
         return v, e.value, n.value, i.value
 
 
-def structured_cfg(n=(16, 16, 16), ds=50.0, acf=0, cl=(2000.0, 2000.0, 2000.0), sigma=0.05, hurst=0.25,
-                   nharm=256, dh=None, alpha=0.0, beta=0.0, gamma=0.0, fs=None, poi=None, mute=-999.0,
-                   taper=-999.0, rescale=0, real=np.float32):
-    """Resolved structured configuration with the reference defaults
-    (src/scarflib.f90:132-158): nc = (fs-1)*ds, fc = (fe-1)*ds, dh = ds."""
-    nd = 3 if cl[2] > 0 else 2
-    fs = tuple(fs) if fs is not None else (1, 1, 1)
-    fe = tuple(fs[i] + n[i] - 1 for i in range(3))
-    r = real
-    nc = tuple(float(r(r(fs[i] - 1) * r(ds))) for i in range(3))
-    fc = tuple(float(r(r(fe[i] - 1) * r(ds))) for i in range(3))
-    return dict(nd=nd, structured=True, fs=fs, fe=fe, ds=float(r(ds)), dh=float(r(dh if dh else ds)), acf=acf,
-                cl=tuple(float(r(c)) for c in cl), sigma=float(r(sigma)), hurst=float(r(hurst)), nc=nc, fc=fc,
-                alpha=float(r(alpha)), beta=float(r(beta)), gamma=float(r(gamma)), nharm=nharm, poi=poi,
-                mute=float(r(mute)), taper=float(r(taper)), rescale=rescale)
+# resolved configurations are built by the package (scarf3d_amd.workloads)
+from scarf3d_amd.workloads import structured_cfg  # noqa: E402,F401

@@ -100,6 +100,37 @@ def test_parallel_draw_consumes_the_same_stream(oracle, lib_f, lib_d, flavour, t
         assert np.array_equal(h1, h2)
 
 
+@pytest.mark.parametrize("flavour", ["f", "d"])
+@pytest.mark.parametrize("chunk", [1, 7, 128, 640])
+def test_resumable_draw_is_cut_invariant(oracle, lib_f, lib_d, flavour, chunk):
+    """The streamed execute draws in chunks of kept harmonics, resuming the
+    walk where it stopped; cut every `chunk` kept harmonics (and with dh <
+    ds, so that harmonics above kc are dropped between and at the cuts), the
+    draw equals the oracle's uncut loop."""
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
+    cfg = structured_cfg(n=(64, 64, 64), ds=20.0, dh=8.0, hurst=0.3, cl=(3000.0,) * 3, nharm=1300, real=real)
+    p = oracle.params(cfg)
+    s = oracle.setup(p, flavour)
+    init_product(lib, cfg)
+    assert lib.set_option("draw_threads", 4) == 0
+    assert lib.set_option("draw_chunk", chunk) == 0
+    h1, u1, n1 = oracle.draw(p, s, 4321, flavour)
+    h2, u2, n2 = lib.draw_harmonics(4321, 1300)
+    assert 0 < u1.sum() < 1300  # some harmonics dropped above kc
+    assert n1 == n2 and np.array_equal(u1, u2) and np.array_equal(h1, h2)
+
+
+def test_stream_chunk_option_checked(lib_f):
+    """Streamed chunks start at multiples of the field kernels' 128-harmonic
+    promotion period: other chunk sizes are refused."""
+    init_product(lib_f, structured_cfg(nharm=64))
+    assert lib_f.set_option("stream_chunk", 100) != 0
+    assert "multiple of 128" in lib_f.last_error()
+    assert lib_f.set_option("stream_chunk", 256) == 0
+    assert lib_f.set_option("stream_draw", 0) == 0
+
+
 def test_draw_options_rejected_when_negative(lib_f):
     init_product(lib_f, structured_cfg(nharm=64))
     assert lib_f.set_option("draw_threads", -1) != 0
