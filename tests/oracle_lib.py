@@ -207,3 +207,24 @@ class RefQuadpack:
 
 # resolved configurations are built by the package (scarf3d_amd.workloads)
 from scarf3d_amd.workloads import structured_cfg  # noqa: E402,F401
+
+
+def epilogue_at(vals, flat, cfg, mean, var):
+    """Rescale (src/scarflib_srm.f90:897-912: field * sigma/sqrt(var) - mean)
+    and the taper/mute of every POI in order (scarflib_common.f90:881-941) at
+    the structured points `flat`, in FP64, given the FP64 field's moments."""
+    out = np.array(vals, dtype=np.float64)
+    if cfg.get("rescale") == 1:
+        out = out * (cfg["sigma"] / np.sqrt(var)) - mean
+    poi = cfg.get("poi")
+    if poi is not None and cfg.get("structured", True):
+        n = [cfg["fe"][a] - cfg["fs"][a] + 1 for a in range(3)]
+        idx = [flat % n[0], (flat // n[0]) % n[1], flat // (n[0] * n[1])]
+        pts = np.stack([(idx[a] + cfg["fs"][a] - 1) * cfg["ds"] for a in range(3)])
+        mute, taper = cfg["mute"], cfg["taper"]
+        for q in range(np.asarray(poi).shape[1]):
+            d = np.sqrt(((pts - np.asarray(poi)[:, q:q + 1]) ** 2).sum(axis=0))
+            f = np.where(d <= mute, 0.0, np.where(d <= mute + taper,
+                                                  0.5 - 0.5 * np.cos(np.pi * (d - mute) / taper), 1.0))
+            out = out * f
+    return out

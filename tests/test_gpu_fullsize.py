@@ -7,12 +7,16 @@ against the FP64 CPU oracle at 256 sampled points, and the fused statistics
 against the field they were computed from.
 
 1024 x 1024 x 512, 50,000 harmonics, 16 points of interest with mute and
-taper, rescaled to sigma: the split-bf16 field against the FP64 kernel's, each
-after its own statistics, rescale and taper (bar 1e-5 sigma).
+taper, rescaled to sigma: the split-bf16 and scaled-fp16 fields against the
+FP64 kernel's FP64 field, each after its own statistics, rescale and taper
+(bar 1e-5 sigma), and the FP64 CPU oracle at 256 sampled points through the
+same rescale and taper.
 
 128 x 128 x 128, 5,000 harmonics with the double library, and 2,000,000
-scattered points with 10,000 harmonics: every kernel against the FP64 kernel
-at every point, and the FP64 kernel against the oracle at 256 sampled points.
+scattered points with 10,000 harmonics (FP64 coordinates through the double
+library, and rounded to float through the float library): every kernel
+against the FP64 kernel at every point, and against the oracle at 256
+sampled points.
 
 The 64-member ensemble: every member equal to its single run bit for bit."""
 import numpy as np
@@ -78,7 +82,10 @@ def test_headline_configuration_full_size(oracle, lib_f):
     assert err64 <= 1e-6
 
 
-def test_large_grid_with_taper_and_rescale_full_size(lib_f):
+def test_large_grid_with_taper_and_rescale_full_size(oracle, lib_f):
+    from oracle_lib import epilogue_at
+    from scarf3d_amd.hip import Hip
+
     n = (1024, 1024, 512)
     ds = 20.0
     rng = np.random.default_rng(16)
@@ -87,15 +94,40 @@ def test_large_grid_with_taper_and_rescale_full_size(lib_f):
     cfg = structured_cfg(n=n, ds=ds, acf=0, hurst=0.2, cl=(6000.0, 4000.0, 1500.0), sigma=0.08, nharm=50000,
                          rescale=1, poi=poi, mute=20.0, taper=500.0, real=np.float32)
     seed = 1234
-    print("[case] 1024 x 1024 x 512 x 50000, 16 POIs, rescale", flush=True)
+    print("[case] 1024 x 1024 x 512 x 50000, 16 POIs, rescale: FP64 kernel (FP64 output)", flush=True)
     init_product(lib_f, cfg)
-    ref, st64 = _run(lib_f, "direct_f64", seed)
-    field, st = _run(lib_f, "mfma_bf16x3", seed)
-    err = _max_abs_diff(field, ref) / cfg["sigma"]
-    print(f"  mfma_bf16x3 vs FP64 kernel after rescale and taper: max|err|/sigma = {err:.3e}", flush=True)
-    assert err <= 1e-5
-    # rescaled: stats(3) reports sigma and stats(4) zero for both
-    assert st[2] == st64[2] and st[3] == st64[3] == 0.0
+    # the FP64 kernel's FP64 field after its own FP64 statistics, rescale and taper
+    hip = Hip()
+    lib_f.srm_draw(seed)
+    d_ref = hip.empty(int(np.prod(n)), np.float64)
+    st64 = lib_f.srm_run_fp64(d_ref.ptr)  # {count, mean, variance, sigma_d} before the epilogue
+    ref = d_ref.numpy()
+    d_ref.free()
+    for kernel in ("mfma_bf16x3", "mfma_f16x2"):
+        field, st = _run(lib_f, kernel, seed)
+        err = _max_abs_diff(field, ref) / cfg["sigma"]
+        print(f"  {kernel} vs FP64 kernel after rescale and taper: max|err|/sigma = {err:.3e}", flush=True)
+        assert err <= 1e-5
+        # rescaled: stats(3) reports sigma and stats(4) zero
+        assert st[2] == np.float32(cfg["sigma"]) and st[3] == 0.0
+        if kernel == "mfma_bf16x3":
+            bf = _flat(field)
+        del field
+    # the FP64 CPU oracle at 256 sampled points, through the reference's
+    # rescale and taper with the FP64 field's moments
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "f")
+    harm, used, _ = oracle.draw(p, s, seed, "f")
+    rng = np.random.default_rng(5)
+    ii, jj, kk = (rng.integers(0, n[a], 256) for a in range(3))
+    flat = ii + n[0] * (jj + n[1] * kk)
+    vals = epilogue_at(oracle.field_at(p, s, harm, used, ii, jj, kk), flat, cfg, st64[1], st64[2])
+    err64 = np.abs(ref[flat] - vals).max() / cfg["sigma"]
+    errbf = np.abs(bf[flat].astype(np.float64) - vals).max() / cfg["sigma"]
+    print(f"  oracle at 256 points after rescale and taper: FP64 kernel {err64:.3e}, mfma_bf16x3 {errbf:.3e} sigma",
+          flush=True)
+    assert err64 <= 1e-6 and errbf <= 1e-5
+    assert np.count_nonzero(vals == 0.0) == np.count_nonzero(ref[flat] == 0.0)  # muted points
 
 
 def test_plain_128_double_library_full_size(oracle, lib_d):
@@ -122,26 +154,33 @@ def test_plain_128_double_library_full_size(oracle, lib_d):
     assert err64 <= 1e-11
 
 
-def test_points_configuration_full_size(oracle, lib_f):
+@pytest.mark.parametrize("flavour", ["f", "d"])
+def test_points_configuration_full_size(oracle, lib_f, lib_d, flavour):
+    """2,000,000 points with FP64 coordinates through the double library
+    (the configuration's coordinates), and the same points rounded to float
+    through the float library; the FP32 direct kernel in both."""
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
     cfg = unstructured_cfg(npts=2_000_000, box=(10000.0, 10000.0, 5000.0), dh=50.0, acf=1,
-                           cl=(3000.0, 3000.0, 1000.0), sigma=0.1, hurst=0.25, nharm=10000, seed=7, real=np.float32)
+                           cl=(3000.0, 3000.0, 1000.0), sigma=0.1, hurst=0.25, nharm=10000, seed=7, real=real)
     seed = 7
-    print("[case] 2,000,000 points x 10000", flush=True)
-    init_product(lib_f, cfg)
-    ref, _ = _run(lib_f, "direct_f64", seed)
-    field, _ = _run(lib_f, "direct_f32", seed)
+    print(f"[case] 2,000,000 points x 10000, library {flavour}", flush=True)
+    init_product(lib, cfg)
+    ref, _ = _run(lib, "direct_f64", seed)
+    field, _ = _run(lib, "direct_f32", seed)
     err = _max_abs_diff(field, ref) / cfg["sigma"]
     print(f"  direct_f32 vs FP64 kernel, every point: max|err|/sigma = {err:.3e}", flush=True)
     assert err <= 1e-5
     p = oracle.params(cfg)
-    s = oracle.setup(p, "f")
-    harm, used, _ = oracle.draw(p, s, seed, "f")
+    s = oracle.setup(p, flavour)
+    harm, used, _ = oracle.draw(p, s, seed, flavour)
     flat = np.random.default_rng(5).integers(0, p.npts, 256)
     sub = dict(cfg, x=cfg["x"][flat], y=cfg["y"][flat], z=cfg["z"][flat])  # same parameters, sampled points
     vals = oracle.field_unstructured(oracle.params(sub), s, harm, used)
     err64 = np.abs(_flat(ref)[flat].astype(np.float64) - vals).max() / cfg["sigma"]
-    print(f"  FP64 kernel vs oracle at 256 points: max|err|/sigma = {err64:.3e}", flush=True)
-    assert err64 <= 1e-6
+    errf = np.abs(_flat(field)[flat].astype(np.float64) - vals).max() / cfg["sigma"]
+    print(f"  oracle at 256 points: FP64 kernel {err64:.3e}, direct_f32 {errf:.3e} sigma", flush=True)
+    assert err64 <= (1e-6 if flavour == "f" else 1e-11) and errf <= 1e-5
 
 
 def test_ensemble_64_full_size(lib_f):
@@ -149,11 +188,11 @@ def test_ensemble_64_full_size(lib_f):
     harmonics; one persistent launch over mixed x tiles, items dealt to the
     least-loaded block): every member's field and statistics equal its single
     run bit for bit."""
-    import bench
     from scarf3d_amd.hip import Hip
+    from scarf3d_amd.workloads import ensemble_members
 
     hip = Hip()
-    members = bench.ensemble_members()
+    members = ensemble_members()
     bufs = [hip.empty(int(np.prod(m["fe"])), np.float32) for m in members]
     print("[case] ensemble of 64, mfma_bf16x3", flush=True)
     stats = lib_f.ensemble_execute(members, [b.ptr for b in bufs], "mfma_bf16x3")

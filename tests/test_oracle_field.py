@@ -11,7 +11,11 @@ would fail:
 - mean/variance equal numpy on random data, and the Chan merge of split
   sets equals the statistics of the whole;
 - the taper leaves points beyond mute+taper unchanged, zeroes points
-  within mute, and rises from 0 to 1 across the taper ring.
+  within mute, and rises from 0 to 1 across the taper ring;
+- one harmonic is a closed-form sinusoid: a pure sine for (a, b) = (1, 0)
+  and a pure cosine for (0, 1) at zero angles, and at alpha = 90 degrees a
+  wave along y whose phase uses the hand-computed bar;
+- the sampled rescale + taper used at full size equals the oracle pipeline.
 """
 import ctypes as C
 
@@ -59,6 +63,46 @@ def test_block_offset_shifts_the_field(oracle):
     f1 = oracle.field_structured(p1, s1, harm, used)
     f2 = oracle.field_structured(p2, s2, harm, used)
     assert np.allclose(f2, f1[2:, 1:, 3:], rtol=0, atol=1e-15)
+
+
+def _grid_coords(p):
+    """global coordinates (i + fs - 2) * ds of the block, 0-based i (src/scarflib_srm.f90:796-800)"""
+    ax = [(np.arange(p.fe[a] - p.fs[a] + 1) + p.fs[a] - 1) * p.ds for a in range(3)]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+@pytest.mark.parametrize("ab,wave", [((1.0, 0.0), np.sin), ((0.0, 1.0), np.cos), ((-0.6, 0.8), None)])
+def test_single_harmonic_closed_form(oracle, ab, wave):
+    """One harmonic at zero angles (M = I, bar = 0) is the sinusoid
+    scaling * (a sin(v.p) + b cos(v.p)) of src/scarflib_srm.f90:802-805: a
+    pure sine for (a, b) = (1, 0), a pure cosine for (0, 1). A swapped a/b,
+    a sign error in the phase or a transposed axis fails here."""
+    cfg, p, s = _setup(oracle, n=(9, 7, 5), fs=(2, 3, 1), nharm=1)
+    v = np.array([1.3e-3, -0.7e-3, 2.1e-3])  # rad/m
+    harm = np.array([[*v, *ab]])
+    f = oracle.field_structured(p, s, harm, np.ones(1, np.uint8))
+    x, y, z = _grid_coords(p)
+    t = v[0] * x + v[1] * y + v[2] * z
+    expect = s.scaling * (wave(t) if wave else ab[0] * np.sin(t) + ab[1] * np.cos(t))
+    assert np.allclose(f, expect, rtol=0, atol=1e-15 * s.scaling)
+    if wave is np.sin:  # f(0) = 0 only for the sine: at the origin of the model
+        cfg0, p0, s0 = _setup(oracle, n=(3, 3, 3), nharm=1)
+        assert oracle.field_structured(p0, s0, harm, np.ones(1, np.uint8))[0, 0, 0] == 0.0
+
+
+def test_single_harmonic_alpha_ninety_by_hand(oracle):
+    """alpha = 90 degrees: M p = (-y, x, z) and bar = h - M h = (hx + hy,
+    hy - hx, 0) with h the half-span (src/scarflib_srm.f90:677-700), so a
+    harmonic v = (v1, 0, 0) has the phase v1 (hx + hy - y): the field does not
+    depend on x, and b = 1 gives scaling * cos(v1 (hx + hy - y))."""
+    cfg, p, s = _setup(oracle, n=(6, 8, 4), alpha=90.0, nharm=1)
+    h = (np.array(cfg["fc"]) - np.array(cfg["nc"])) / 2
+    assert np.allclose(np.array(s.bar[:]), [h[0] + h[1], h[1] - h[0], 0.0], rtol=0, atol=1e-12)
+    v1 = 2.3e-3
+    f = oracle.field_structured(p, s, np.array([[v1, 0.0, 0.0, 0.0, 1.0]]), np.ones(1, np.uint8))
+    _, y, _ = _grid_coords(p)
+    assert np.allclose(f, s.scaling * np.cos(v1 * (h[0] + h[1] - y)), rtol=0, atol=1e-14 * s.scaling)
+    assert np.allclose(f, f[:1, :, :], rtol=0, atol=1e-14 * s.scaling)  # constant along x
 
 
 def test_ensemble_variance_at_a_point(oracle):
@@ -147,3 +191,26 @@ def test_taper_profile(oracle, flavour, dtype):
     # monotone in distance across the ring
     order = np.argsort(d[ring])
     assert np.all(np.diff(f[ring][order]) >= -1e-6)
+
+
+def test_sampled_epilogue_equals_oracle_pipeline(oracle):
+    """epilogue_at (the rescale and the taper of every POI, in FP64 at sampled
+    points, used where the whole-field oracle would take hours) equals the
+    oracle's own pipeline (orc_srm: statistics, rescale, taper) on a small
+    grid, at every point."""
+    from oracle_lib import epilogue_at
+
+    poi = np.array([[300.0, 400.0, 200.0], [900.0, 500.0, 600.0], [100.0, 1000.0, 50.0]]).T
+    cfg = structured_cfg(n=(24, 22, 14), nharm=128, poi=poi, mute=60.0, taper=300.0, rescale=1, real=np.float64)
+    p = oracle.params(cfg)
+    s = oracle.setup(p, "d")
+    harm, used, _ = oracle.draw(p, s, 5, "d")
+    f64 = oracle.field_structured(p, s, harm, used).reshape(-1, order="F")
+    mean = f64.mean()
+    var = ((f64 - mean) ** 2).sum() / (f64.size - 1)
+    flat = np.arange(f64.size)
+    mine = epilogue_at(f64, flat, cfg, mean, var)
+    ref, _ = oracle.srm(p, 5, "d")
+    assert np.count_nonzero(ref == 0.0) > 0  # some points muted
+    assert np.array_equal(mine == 0.0, ref == 0.0)
+    assert np.abs(mine - ref).max() <= 1e-12 * cfg["sigma"]
