@@ -117,6 +117,35 @@ def build_standalone(verbose: bool = False) -> list[Path]:
     return outs
 
 
+CC = os.environ.get("CC_SCARF", "gcc")
+
+
+def build_drivers(verbose: bool = False) -> list[Path]:
+    """The C and C++ drop-in drivers (tests/c/srm_driver.c,
+    tests/cpp/srm_driver.cpp) for both flavours: callers written against
+    include/scarf3d.h as the reference's examples are, linked with the
+    library."""
+    outs = []
+    jobs = []
+    for flav, defs, libname in (("f", [], "scarf3df"), ("d", ["-DDOUBLE_PREC"], "scarf3d")):
+        c_src, cpp_src = ROOT / "tests" / "c" / "srm_driver.c", ROOT / "tests" / "cpp" / "srm_driver.cpp"
+        if c_src.exists():
+            jobs.append(([CC, "-std=c11", "-O2", "-Wall", *defs, "-I", str(ROOT / "include"), str(c_src)],
+                         LIB / f"srm_driver_c_{flav}", libname))
+        if cpp_src.exists():
+            jobs.append(([CXX, "-std=c++17", "-O2", "-Wall", *defs, "-I", str(ROOT / "include"), str(cpp_src)],
+                         LIB / f"srm_driver_cpp_{flav}", libname))
+    for head, exe, libname in jobs:
+        cmd = [*head, "-L", str(LIB), f"-l{libname}", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"driver build failed:\n{res.stdout}{res.stderr}")
+        outs.append(exe)
+    return outs
+
+
 def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
     """Compile every HIP translation unit for gfx950 and link both libraries."""
     BUILD.mkdir(exist_ok=True)
@@ -134,6 +163,7 @@ def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
         os.replace(tmp, out)
         outs.append(out)
     outs.extend(build_standalone(verbose))
+    outs.extend(build_drivers(verbose))
     outs.extend(build_fortran(verbose))
     return outs
 

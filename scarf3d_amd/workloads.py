@@ -57,7 +57,8 @@ def unstructured_cfg(npts=2000, box=(10000.0, 10000.0, 5000.0), dh=50.0, acf=1, 
 def init_library(lib, cfg: dict) -> None:
     """Initialize `lib` (scarf3d_amd.Library) so that its object holds
     exactly `cfg`: options are passed only where the presence rules of
-    src/c_binding.c:89-103 forward them."""
+    src/c_binding.c:89-103 forward them (nc/fc of a structured grid only when
+    they differ from the block's own extents)."""
     nd = cfg["nd"]
     opts = dict(method=1, hurst=cfg.get("hurst", 0.0), alpha=cfg.get("alpha", 0.0), beta=cfg.get("beta", 0.0),
                 gamma=cfg.get("gamma", 0.0), rescale=cfg.get("rescale", 0))
@@ -72,6 +73,14 @@ def init_library(lib, cfg: dict) -> None:
     if cfg.get("structured", True):
         if cfg["dh"] != cfg["ds"]:
             opts["dh"] = cfg["dh"]
+        # model extents other than the block's own (a block of a larger grid,
+        # as each MPI rank of the reference holds): passed as nc/fc
+        r = lib.dtype
+        own_nc = [float(r(r(cfg["fs"][i] - 1) * r(cfg["ds"]))) for i in range(nd)]
+        own_fc = [float(r(r(cfg["fe"][i] - 1) * r(cfg["ds"]))) for i in range(nd)]
+        if list(cfg["nc"][:nd]) != own_nc or list(cfg["fc"][:nd]) != own_fc:
+            opts["nc"] = list(cfg["nc"][:nd])
+            opts["fc"] = list(cfg["fc"][:nd])
         o = lib.opt(**opts)
         lib.cart_initialize(nd, cfg["fs"][:nd], cfg["fe"][:nd], cfg["ds"], cfg["acf"], cfg["cl"][:nd], cfg["sigma"],
                             o)
