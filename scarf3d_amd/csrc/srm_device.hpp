@@ -116,14 +116,34 @@ public:
     int last_nbx() const { return last_nbx_; }
     int last_tk() const { return last_tk_; }
 
+    // device bytes held by the library's grow-only buffers
+    size_t reserved_bytes() const { return reserved_; }
+
 private:
+    // Grows *ptr to `bytes` (keeps it when large enough). With
+    // SCARF_DEVICE_MEMORY_LIMIT=<bytes> set, a call that would leave the
+    // buffers holding more than that fails with an error instead, as a
+    // failed hipMalloc does: the allocation-failure paths can be exercised.
     bool reserve(void **ptr, size_t &cap, size_t bytes, std::string &err);
+    // the same on a typed member: the pointer is written back whatever the
+    // outcome (after a failed hipMalloc it is null, its capacity 0)
+    template <class T>
+    bool reserve(T *&ptr, size_t &cap, size_t bytes, std::string &err)
+    {
+        void *p = ptr;
+        const bool ok = reserve(&p, cap, bytes, err);
+        ptr = static_cast<T *>(p);
+        return ok;
+    }
+    size_t reserved_ = 0;
     // statistics, epilogue, field hand-over and timings after the field
-    // kernels; slab_first/slab_points name the parts whose events are slab_ev_
+    // kernels; slab_first/slab_points name the parts whose events are
+    // slab_ev_. part/stat/poi: device room for the partials, the statistics
+    // and the points of interest (null: the library's own buffers)
     template <class Real>
     bool finish(const Geometry &g, const Epilogue &ep, Real *d_field, int nparts, double *stats_out,
                 const std::vector<long> &slab_first, const std::vector<long> &slab_points, std::string &err,
-                const SlabHook *on_slab);
+                const SlabHook *on_slab, double *part = nullptr, double *stat = nullptr, double *poi = nullptr);
 
     int device_ = -1;
     hipStream_t stream_ = nullptr;

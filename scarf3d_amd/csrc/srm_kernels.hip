@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 
 #include "srm_device.hpp"
 #include "srm_mfma.hpp"
@@ -496,21 +498,33 @@ bool Device::ensure(std::string &err)
 
 bool Device::reserve(void **ptr, size_t &cap, size_t bytes, std::string &err)
 {
-    if (bytes <= cap && *ptr) return true;
+    const bool grow = !(bytes <= cap && *ptr);
+    if (const char *e = std::getenv("SCARF_DEVICE_MEMORY_LIMIT")) {
+        const double limit = std::atof(e);
+        const size_t after = reserved_ + (grow ? bytes - std::min(bytes, cap) : 0);
+        if (limit > 0 && double(after) > limit) {
+            err = "device memory limit SCARF_DEVICE_MEMORY_LIMIT=" + std::string(e) + " bytes: the library holds " +
+                  std::to_string(reserved_) + " and would need " + std::to_string(after);
+            return false;
+        }
+    }
+    if (!grow) return true;
     if (*ptr) {
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipFree(*ptr));
         *ptr = nullptr;
+        reserved_ -= cap;
         cap = 0;
     }
     HIP_TRY(hipMalloc(ptr, std::max<size_t>(bytes, 256)));
     cap = bytes;
+    reserved_ += cap;
     return true;
 }
 
 void *Device::field_buffer(size_t bytes, std::string &err)
 {
-    if (!reserve(&d_field_, field_cap_, bytes, err)) return nullptr;
+    if (!reserve(d_field_, field_cap_, bytes, err)) return nullptr;
     return d_field_;
 }
 
@@ -526,9 +540,7 @@ bool Device::upload_table(const HarmonicTable &t, std::string &err)
         rec[3 * size_t(np_) + l] = t.phase[l];
         rec[4 * size_t(np_) + l] = t.amp[l];
     }
-    void *p = d_harm_;
-    if (!reserve(&p, harm_cap_, rec.size() * sizeof(double), err)) return false;
-    d_harm_ = static_cast<double *>(p);
+    if (!reserve(d_harm_, harm_cap_, rec.size() * sizeof(double), err)) return false;
     HIP_TRY(hipMemcpyAsync(d_harm_, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     nharm_ = np_;
@@ -546,9 +558,7 @@ bool Device::upload_points(const Real *x, const Real *y, const Real *z, long n, 
         pts[n + i] = y[i];
         pts[2 * n + i] = z ? static_cast<double>(z[i]) : 0.0;
     }
-    void *p = d_pts_;
-    if (!reserve(&p, pts_cap_, pts.size() * sizeof(double), err)) return false;
-    d_pts_ = static_cast<double *>(p);
+    if (!reserve(d_pts_, pts_cap_, pts.size() * sizeof(double), err)) return false;
     HIP_TRY(hipMemcpyAsync(d_pts_, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     npts_ = n;
@@ -591,13 +601,9 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
             last_nbx_ = plan.nbx;
             last_tk_ = plan.tk;
             plan.zscale = mfma_zscale(kernel, amp_max_);
-            void *p = d_tab_;
-            if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;
-            d_tab_ = p;
+            if (!reserve(d_tab_, tab_cap_, plan.table_bytes, err)) return false;
             nparts = plan.nparts;
-            void *q = d_part_;
-            if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
-            d_part_ = static_cast<double *>(q);
+            if (!reserve(d_part_, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
             if (!mfma_tables(plan, g, d_harm_, np_, d_tab_, stream_, err)) return false;
             HIP_TRY(hipEventRecord(ev_[1], stream_));
             if (on_slab && !ep.active() && plan.tiles_k > 1) {
@@ -624,15 +630,11 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
             const bool f64 = kernel == kKernelDirectF64;
             const size_t elt = f64 ? sizeof(double) : sizeof(float);
             const size_t tab_bytes = size_t(np_) * (size_t(g.nx) + g.ny + g.nz) * elt + size_t(np_) * sizeof(float);
-            void *p = d_tab_;
-            if (!reserve(&p, tab_cap_, tab_bytes, err)) return false;
-            d_tab_ = p;
+            if (!reserve(d_tab_, tab_cap_, tab_bytes, err)) return false;
             const int tiles_i = int((g.nx + kDirB - 1) / kDirB);
             const long tiles_c = (long(g.ny) * g.nz + kDirB - 1) / kDirB;
             nparts = int(tiles_i * tiles_c);
-            void *q = d_part_;
-            if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
-            d_part_ = static_cast<double *>(q);
+            if (!reserve(d_part_, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
             const long tab_elems = long(np_) * (g.nx + g.ny + g.nz);
             const int tblocks = int(std::min<long>((tab_elems + 255) / 256, 65536));
             if (f64) {
@@ -670,9 +672,7 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
         }
         HIP_TRY(hipEventRecord(ev_[1], stream_));
         nparts = int((npts + 256 * kPtsP - 1) / (256 * kPtsP));
-        void *q = d_part_;
-        if (!reserve(&q, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
-        d_part_ = static_cast<double *>(q);
+        if (!reserve(d_part_, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
         if (kernel == kKernelDirectF64)
             direct_points<Real, true><<<nparts, 256, 0, stream_>>>(d_pts_, d_pts_ + npts, d_pts_ + 2 * npts, npts,
                                                                    d_harm_, np_, scaling, d_field, d_part_);
@@ -687,36 +687,38 @@ bool Device::run(const Geometry &g, int kernel, double scaling, const Epilogue &
 template <class Real>
 bool Device::finish(const Geometry &g, const Epilogue &ep, Real *d_field, int nparts, double *stats_out,
                     const std::vector<long> &slab_first, const std::vector<long> &slab_points, std::string &err,
-                    const SlabHook *on_slab)
+                    const SlabHook *on_slab, double *part, double *stat, double *poi)
 {
     const long npts = g.points();
+    if (!part) part = d_part_;
+    if (!stat) stat = d_stat_;
     HIP_TRY(hipEventRecord(ev_[2], stream_));
-    stats_finalize<Real><<<1, 256, 0, stream_>>>(d_part_, nparts, ep.sigma, d_stat_);
+    stats_finalize<Real><<<1, 256, 0, stream_>>>(part, nparts, ep.sigma, stat);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev_[3], stream_));
 
     if (ep.active()) {
         const int npoi = static_cast<int>(ep.poi3.size() / 3);
         if (npoi > 0) {
-            void *p = d_poi_;
-            if (!reserve(&p, poi_cap_, ep.poi3.size() * sizeof(double), err)) return false;
-            d_poi_ = static_cast<double *>(p);
-            HIP_TRY(hipMemcpyAsync(d_poi_, ep.poi3.data(), ep.poi3.size() * sizeof(double), hipMemcpyHostToDevice,
+            if (!poi) {
+                if (!reserve(d_poi_, poi_cap_, ep.poi3.size() * sizeof(double), err)) return false;
+                poi = d_poi_;
+            }
+            HIP_TRY(hipMemcpyAsync(poi, ep.poi3.data(), ep.poi3.size() * sizeof(double), hipMemcpyHostToDevice,
                                    stream_));
         }
         const int blocks = int(std::min<long>((npts + 255) / 256, 16384));
         if (g.structured)
             epilogue_grid<Real><<<blocks, 256, 0, stream_>>>(d_field, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.ds,
-                                                             d_stat_, ep.rescale == 1, d_poi_, npoi, ep.mute,
-                                                             ep.taper);
+                                                             stat, ep.rescale == 1, poi, npoi, ep.mute, ep.taper);
         else
             epilogue_points<Real><<<blocks, 256, 0, stream_>>>(d_field, npts, d_pts_, d_pts_ + npts,
-                                                               d_pts_ + 2 * npts, d_stat_, ep.rescale == 1, d_poi_,
-                                                               npoi, ep.mute, ep.taper);
+                                                               d_pts_ + 2 * npts, stat, ep.rescale == 1, poi, npoi,
+                                                               ep.mute, ep.taper);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ev_[4], stream_));
-    HIP_TRY(hipMemcpyAsync(h_stat_, d_stat_, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(h_stat_, stat, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     for (size_t c = 0; c < slab_first.size(); ++c) {
         HIP_TRY(hipEventSynchronize(slab_ev_[c]));
         if (!(*on_slab)(slab_first[c], slab_points[c], err)) return false;
@@ -793,9 +795,7 @@ bool Device::run_streamed(const Geometry &g, int kernel, int nharm, double scali
     last_tk_ = plan.tk;
     plan.zscale = 1.0;  // a chunk that would need another scale restarts on run()
     const size_t rec_bytes = size_t(5) * cap * sizeof(double);
-    void *p = d_harm_;
-    if (!reserve(&p, harm_cap_, rec_bytes, err)) return false;
-    d_harm_ = static_cast<double *>(p);
+    if (!reserve(d_harm_, harm_cap_, rec_bytes, err)) return false;
     nharm_ = 0;  // the streamed record is not a resident table for run()
     if (h_rec_cap_ < rec_bytes) {
         HIP_TRY(hipStreamSynchronize(stream_));
@@ -805,13 +805,9 @@ bool Device::run_streamed(const Geometry &g, int kernel, int nharm, double scali
         HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&h_rec_), rec_bytes, hipHostMallocDefault));
         h_rec_cap_ = rec_bytes;
     }
-    p = d_tab_;
-    if (!reserve(&p, tab_cap_, plan.table_bytes, err)) return false;
-    d_tab_ = p;
+    if (!reserve(d_tab_, tab_cap_, plan.table_bytes, err)) return false;
     const int nparts = plan.nparts;
-    p = d_part_;
-    if (!reserve(&p, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
-    d_part_ = static_cast<double *>(p);
+    if (!reserve(d_part_, part_cap_, size_t(3) * nparts * sizeof(double), err)) return false;
 
     timings_ = Timings{};
     HIP_TRY(hipEventRecord(ev_[0], stream_));

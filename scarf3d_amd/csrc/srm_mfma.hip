@@ -205,6 +205,7 @@ struct TableArgs {
     float2 *yt, *zt;
     int np_, nx, nx_pad, ny, nz, ox, oy, oz;
     int h0, nh;          // harmonics [h0, h0 + nh) (multiples of the stage)
+    int t0;              // first harmonic the tables hold (a multiple of the stage)
     double ds, zscale;
 };
 
@@ -218,12 +219,13 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
     E *__restrict__ xt = static_cast<E *>(a.xt);
     const int np_ = a.np_, nx = a.nx, nx_pad = a.nx_pad, ny = a.ny, nz = a.nz;
     const double ds = a.ds;
-    const int h0 = a.h0, nh = a.nh;
+    const int h0 = a.h0, nh = a.nh, t0 = a.t0;
     const long nxi = long(nh / kStageH) * nx_pad, nyi = long(nh) * ny;
     const long total = nxi + nyi + long(nh) * nz;
     for (long e = first; e < total; e += step) {
         if (e < nxi) {
             const int stage = h0 / kStageH + int(e / nx_pad), r = int(e % nx_pad);
+            const int slot = stage - t0 / kStageH;  // the stage's place in the tables
             uint32_t words[P][kStageH];  // harmonic h: (cos, -sin) parts as one little-endian word
 #pragma unroll
             for (int h = 0; h < kStageH; ++h) {
@@ -245,7 +247,7 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
             }
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                uint4 *row = reinterpret_cast<uint4 *>(xt + ((size_t(stage) * P + p) * nx_pad + r) * kStageK);
+                uint4 *row = reinterpret_cast<uint4 *>(xt + ((size_t(slot) * P + p) * nx_pad + r) * kStageK);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     row[q] = uint4{words[p][4 * q], words[p][4 * q + 1], words[p][4 * q + 2], words[p][4 * q + 3]};
@@ -256,7 +258,7 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
             t -= rint(t);
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
-            a.yt[size_t(n) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
+            a.yt[size_t(n - t0) * ny + j] = make_float2(static_cast<float>(cd), static_cast<float>(sd));
         } else {
             const int n = h0 + int((e - nxi - nyi) / nz), k = int((e - nxi - nyi) % nz);
             double t = (harm[2 * np_ + n] * (double(k + a.oz) * ds) + harm[3 * np_ + n]) * kInvTwoPi;
@@ -264,7 +266,7 @@ __device__ __forceinline__ void table_items(const TableArgs &a, long first, long
             double sd, cd;
             sincospi(2.0 * t, &sd, &cd);
             const double amp = harm[4 * np_ + n] * a.zscale;
-            a.zt[size_t(n) * nz + k] = make_float2(static_cast<float>(amp * cd), static_cast<float>(amp * sd));
+            a.zt[size_t(n - t0) * nz + k] = make_float2(static_cast<float>(amp * cd), static_cast<float>(amp * sd));
         }
     }
 }
@@ -902,6 +904,7 @@ TableArgs table_args(const MfmaPlan &plan, const Geometry &g, const double *d_ha
     a.zscale = plan.zscale;
     a.h0 = 0;
     a.nh = np_;
+    a.t0 = 0;
     return a;
 }
 
@@ -912,13 +915,15 @@ bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, 
 {
     TableArgs a = table_args(plan, g, d_harm, np_, d_tab);
     if (nh < 0) nh = np_ - h0;
-    if (h0 < 0 || h0 % kStageH != 0 || nh % kStageH != 0 || h0 + nh > np_ || h0 + nh > plan.np_) {
+    const int t0 = plan.tab_s0 * kStageH;
+    if (h0 < t0 || h0 % kStageH != 0 || nh % kStageH != 0 || h0 + nh > np_ || h0 - t0 + nh > plan.np_) {
         err = "mfma_tables: harmonic range outside the table";
         return false;
     }
     if (nh == 0) return true;
     a.h0 = h0;
     a.nh = nh;
+    a.t0 = t0;
     const long total = long(nh / kStageH) * plan.nx_pad + long(nh) * (g.ny + g.nz);  // work items
     const int blocks = int(std::min<long>((total + 255) / 256, 65536));
     switch (scheme_of(plan.kernel)) {
@@ -942,10 +947,10 @@ void launch_grid(const MfmaPlan &plan, const Geometry &g, const char *base, doub
 {
     using E = typename SchemeOf<S>::elt;
     // operand tables from stage s0 on: X is stage-major, Y and Z harmonic-major
-    const E *xt = reinterpret_cast<const E *>(base) + size_t(s0) * SchemeOf<S>::parts * plan.nx_pad * kStageK;
-    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes) + size_t(s0) * kStageH * g.ny;
-    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes) +
-                       size_t(s0) * kStageH * g.nz;
+    const size_t ts = size_t(s0 - plan.tab_s0);
+    const E *xt = reinterpret_cast<const E *>(base) + ts * SchemeOf<S>::parts * plan.nx_pad * kStageK;
+    const float2 *yt = reinterpret_cast<const float2 *>(base + plan.x_bytes) + ts * kStageH * g.ny;
+    const float2 *zt = reinterpret_cast<const float2 *>(base + plan.x_bytes + plan.y_bytes) + ts * kStageH * g.nz;
     const dim3 grid(static_cast<unsigned>(blocks));
 #define SCARF_GRID_ARGS xt, yt, zt, g.nx, g.ny, g.nz, plan.nx_pad, nstages, plan.tiles_x, plan.tj_shift, plan.tk, \
                         plan.tiles_j, scaling, d_field, d_part, first, flags
@@ -978,9 +983,10 @@ bool mfma_field(const MfmaPlan &plan, const Geometry &g, const void *d_tab, doub
         err = "mfma_field: first tile out of range";
         return false;
     }
-    if (nstages < 0) nstages = plan.stages - s0;
-    // a chunk starts at a promotion boundary and holds at least one stage
-    if (s0 < 0 || nstages < 1 || s0 + nstages > plan.stages || s0 % kPromote != 0 ||
+    if (nstages < 0) nstages = plan.stages - (s0 - plan.tab_s0);
+    // a chunk starts at a promotion boundary, holds at least one stage and
+    // lies inside the tables
+    if (s0 < plan.tab_s0 || nstages < 1 || s0 - plan.tab_s0 + nstages > plan.stages || s0 % kPromote != 0 ||
         (flags & ~(kFieldCarryIn | kFieldFinal)) != 0 || ((flags & kFieldCarryIn) == 0) != (s0 == 0) ||
         (((flags & kFieldFinal) == 0) && nstages % kPromote != 0)) {
         err = "mfma_field: stage range or flags do not fit the table";
@@ -1357,6 +1363,16 @@ void TileTuner::save_disk(const Key &k, const TileChoice &c)
 
 // SCARF_TILE=x,j,k (x in {64, 96, 128}, j * k = 256, k in {1, 2, 4, 8})
 // fixes the block tile of single-field runs, e.g. to profile one tile
+bool TileTuner::cached(const Geometry &g, int kernel, size_t real_bytes, TileChoice &out)
+{
+    const Key key{long(kernel), long(real_bytes), long(g.nx), long(g.ny), long(g.nz)};
+    load_disk();
+    const auto it = cache_.find(key);
+    if (it == cache_.end()) return false;
+    out = it->second;
+    return true;
+}
+
 bool TileTuner::forced(TileChoice &out)
 {
     const char *e = std::getenv("SCARF_TILE");

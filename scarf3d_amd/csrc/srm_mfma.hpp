@@ -41,6 +41,10 @@ struct MfmaPlan {
     bool mixed = false;
     int x_full = 0;
     int x_tail[2] = {0, 0};
+    // first stage the operand tables hold: 0 when they hold every stage; a
+    // chunk's start when they hold one chunk of a realization (np_ and the
+    // table sizes are then the chunk's capacity)
+    int tab_s0 = 0;
 };
 
 // Power of two that keeps |Z Y| * 2^12 inside fp16 for the scaled-fp16
@@ -54,7 +58,8 @@ bool mfma_plan(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::stri
 // or 96 points, which cover any multiple of 32 points without padding
 bool mfma_plan_mixed(const Geometry &g, int np_, int kernel, MfmaPlan &plan, std::string &err, int tk = 0);
 // Operand tables of harmonics [h0, h0 + nh) of a record of np_ harmonics per
-// field (all of them when nh < 0); h0 and nh are multiples of 16.
+// field (all of them when nh < 0); h0 and nh are multiples of 16. They are
+// stored from stage plan.tab_s0 on.
 bool mfma_tables(const MfmaPlan &plan, const Geometry &g, const double *d_harm, int np_, void *d_tab,
                  hipStream_t stream, std::string &err, int h0 = 0, int nh = -1);
 
@@ -127,6 +132,9 @@ public:
     template <class Real>
     bool choose_saturated(const Geometry &g, int kernel, hipStream_t stream, TileChoice &out, std::string &err);
     static bool forced(TileChoice &out);  // SCARF_TILE, when set to a valid tile
+    // the choice made earlier for this grid (in this process, or in the
+    // SCARF_TUNE_CACHE file), without timing anything
+    bool cached(const Geometry &g, int kernel, size_t real_bytes, TileChoice &out);
     int tuned_shapes() const { return static_cast<int>(cache_.size()); }
 
 private:

@@ -39,6 +39,7 @@ class Hip:
         L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.hipDeviceSynchronize.argtypes = []
         L.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        L.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.hipGetErrorString.restype = C.c_char_p
         self.L = L
 
@@ -50,6 +51,12 @@ class Hip:
         n = C.c_int(0)
         rc = self.L.hipGetDeviceCount(C.byref(n))
         return n.value if rc == 0 else 0
+
+    def mem_get_info(self):
+        """(free, total) bytes of the current device (hipMemGetInfo)."""
+        free, total = C.c_size_t(0), C.c_size_t(0)
+        self._check(self.L.hipMemGetInfo(C.byref(free), C.byref(total)), "hipMemGetInfo")
+        return free.value, total.value
 
     def synchronize(self):
         self._check(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")

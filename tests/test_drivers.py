@@ -51,6 +51,7 @@ def test_drivers_are_built_and_report_errors(kind, flavour, tmp_path):
 
 
 def _lib_field(lib, nd, opts, points=None):
+    assert lib.set_kernel("auto") == 0  # as in the drivers' own processes
     o = lib.opt(method=1, **opts)
     if points is None:
         n = N[:nd]

@@ -239,3 +239,25 @@ def test_host_copy_equals_device_field(lib_f):
     lib_f.execute_device(11, d.ptr)
     assert np.array_equal(field.reshape(-1, order="F"), d.numpy())
     d.free()
+
+
+def test_device_memory_limit_reports_and_recovers(lib_f, monkeypatch):
+    """Allocation failure, forced from the host: with SCARF_DEVICE_MEMORY_LIMIT
+    below what the library's device buffers hold or need, execute fails with an
+    error (no field, zero statistics) instead of crashing; without the limit
+    the next realization runs and equals a clean one."""
+    cfg = structured_cfg(n=(64, 48, 16), nharm=512, real=np.float32)
+    ref, st_ref = _run(lib_f, cfg, 21, "mfma_bf16x3")
+    lib_f.finalize()
+    monkeypatch.setenv("SCARF_DEVICE_MEMORY_LIMIT", "65536")
+    for kernel in ("mfma_bf16x3", "direct_f32"):
+        _say("device memory limit 64 KiB", kernel)
+        init_product(lib_f, cfg)
+        assert lib_f.set_kernel(kernel) == 0
+        field, stats = lib_f.execute(21)
+        assert "SCARF_DEVICE_MEMORY_LIMIT" in lib_f.last_error()
+        assert not np.any(stats)
+        lib_f.finalize()
+    monkeypatch.delenv("SCARF_DEVICE_MEMORY_LIMIT")
+    field, stats = _run(lib_f, cfg, 21, "mfma_bf16x3")
+    assert np.array_equal(field, ref) and np.array_equal(stats[:4], st_ref[:4])
