@@ -49,6 +49,21 @@ int scarf_execute_device(const int seed, real *d_field, real stats[]);
 int scarf_srm_draw(const int seed, double *host_seconds);
 int scarf_srm_run(real *d_field, real stats[]);
 
+/* scarf_srm_run that allocates nothing. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * operand tables, statistics partials, statistics and points of interest
+ * live in a static device array of 64 MiB that each GPU holds for the life
+ * of the process; the tables are built one chunk of harmonics at a time
+ * (chunks are multiples of 128 harmonics and carry the field's running
+ * sums, so the field equals scarf_srm_run's bit for bit). Takes the resident table of
+ * scarf_srm_draw, and the MFMA kernels of structured grids only; uses the
+ * tile chosen earlier for the grid (or the least-padding one) and tunes
+ * nothing. Refuses, with an error, grids whose operand tables for 128
+ * harmonics do not fit in the array. Every call on one GPU shares the array:
+ * calls that overlap on one GPU race. Returns 0 or -1 (scarf_last_error). */
+int scarf_srm_run_noalloc(real *d_field, real stats[]);
+
 /* FP64 field of the resident table (FP64 kernel, FP64 output, FP64
  * epilogue) into d_field (device, doubles), whatever `real` is: the exact
  * reference against which the faster kernels' errors are measured.

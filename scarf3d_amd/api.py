@@ -30,7 +30,7 @@ EXPORTS = (
     "scarf_set_nharm", "scarf_get_nharm", "scarf_set_kernel", "scarf_get_kernel", "scarf_last_error",
     "scarf_execute_device", "scarf_srm_draw", "scarf_srm_run", "scarf_last_timings", "scarf_stream",
     "scarf_setup_values", "scarf_draw_harmonics", "scarf_get_config", "scarf_set_option", "scarf_srm_run_fp64",
-    "scarf_ensemble_execute", "scarf_ensemble_last_ms", "scarf_last_tile",
+    "scarf_ensemble_execute", "scarf_ensemble_last_ms", "scarf_last_tile", "scarf_srm_run_noalloc",
 )
 
 
@@ -113,6 +113,8 @@ class Library:
         L.scarf_srm_draw.restype = C.c_int
         L.scarf_srm_run.argtypes = [C.c_void_p, rp]
         L.scarf_srm_run.restype = C.c_int
+        L.scarf_srm_run_noalloc.argtypes = [C.c_void_p, rp]
+        L.scarf_srm_run_noalloc.restype = C.c_int
         L.scarf_last_timings.argtypes = [P(C.c_double), C.c_int]
         L.scarf_last_timings.restype = C.c_int
         L.scarf_stream.argtypes = []
@@ -271,6 +273,12 @@ class Library:
         if self.lib.scarf_srm_run(C.c_void_p(d_field_ptr), self._rp(stats)) != 0:
             raise RuntimeError(self.last_error())
         return stats
+
+    def srm_run_noalloc(self, d_field_ptr: int):
+        """scarf_srm_run_noalloc: returns (return code, stats[8]); no device allocation."""
+        stats = np.zeros(8, dtype=self.dtype)
+        rc = self.lib.scarf_srm_run_noalloc(C.c_void_p(d_field_ptr), self._rp(stats))
+        return rc, stats
 
     def srm_run_fp64(self, d_field_ptr: int):
         st = np.zeros(4)

@@ -695,6 +695,25 @@ static int scarf_srm_draw_body(const int seed, double *host_seconds)
     return n;
 }
 
+// scarf_srm_run with no device allocation (Device::run_noalloc): the
+// operand tables, partials, statistics and points of interest go to the
+// library's static device scratch array
+static int scarf_srm_run_noalloc_body(real *d_field, real stats[])
+{
+    State &s = state();
+    if (!d_field) return fail("scarf_srm_run_noalloc: d_field is NULL");
+    if (!s.table_ready || !s.device) return fail("no harmonic table drawn (call scarf_srm_draw first)");
+    const scarf::Geometry g = geometry(s.cfg);
+    const scarf::Epilogue ep = epilogue(s.cfg, s.setup.ndim);
+    const int kernel = resolve_kernel(s.cfg);
+    double st[6] = {};
+    std::string err;
+    if (!s.device->run_noalloc<real>(g, kernel, s.setup.scaling, ep, d_field, st, err)) return fail(err);
+    report_stats(stats, st);
+    s.error.clear();
+    return 0;
+}
+
 static int scarf_srm_run_body(real *d_field, real stats[])
 {
     if (!d_field) return fail("scarf_srm_run: d_field is NULL");
@@ -1053,6 +1072,13 @@ long scarf_draw_harmonics(const int seed, double *harm, unsigned char *used)
 {
     long rc = -1;
     guarded([&] { rc = scarf_draw_harmonics_body(seed, harm, used); });
+    return rc;
+}
+
+int scarf_srm_run_noalloc(real *d_field, real stats[])
+{
+    int rc = -1;
+    guarded([&] { rc = scarf_srm_run_noalloc_body(d_field, stats); });
     return rc;
 }
 

@@ -102,6 +102,22 @@ public:
     // nharm harmonics (the last entry is nharm: draw to the end)
     static std::vector<int> chunk_targets(int nharm, int chunk = 0);
 
+    // run() on the resident table with no device allocation at all: the
+    // operand tables, statistics partials, statistics and points of
+    // interest live in a static device array of kNoAllocScratch bytes
+    // (per GPU, for the life of the process), the tables one chunk of
+    // harmonics at a time (chunks at multiples of 128 harmonics, running
+    // sums carried through the field: the field equals run()'s bit for
+    // bit). MFMA kernels on structured grids only; no tile tuning (a tile
+    // chosen earlier for the grid, else the least-padding one). Refuses, with
+    // an error, grids whose tables for 128 harmonics (or whose partials and
+    // points of interest) do not fit. Every call on one GPU uses the same
+    // array: calls that overlap on one GPU race.
+    static constexpr size_t kNoAllocScratch = size_t(64) << 20;
+    template <class Real>
+    bool run_noalloc(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
+                     double *stats_out, std::string &err);
+
     hipStream_t stream() const { return stream_; }
     const Timings &timings() const { return timings_; }
 
@@ -150,6 +166,7 @@ private:
     hipEvent_t ev_[6] = {};
     static constexpr int kSlabs = 8;  // field parts handed out while the rest is computed
     hipEvent_t slab_ev_[kSlabs] = {};
+    void *scratch_ = nullptr;  // this device's copy of the no-alloc scratch array
     int nharm_ = 0;
     double amp_max_ = 0;  // largest amplitude of the resident table
     std::unique_ptr<TileTuner> tuner_;

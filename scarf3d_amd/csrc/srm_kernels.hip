@@ -443,6 +443,14 @@ bool Epilogue::active() const
     return !poi3.empty() && (mute + taper > 0.0 || mute >= 0.0);
 }
 
+// Scratch of Device::run_noalloc: operand tables, statistics partials,
+// statistics and points of interest, for callers that cannot allocate device
+// memory. Each GPU the library runs on holds its own copy for the life of
+// the process. In host code the array's name is not a device address: host
+// code uses the address hipGetSymbolAddress gives (Device::scratch_), taken
+// once per device in Device::ensure.
+static __device__ __align__(256) unsigned char scratch[Device::kNoAllocScratch];
+
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         const hipError_t e_ = (expr);                                                        \
@@ -493,6 +501,8 @@ bool Device::ensure(std::string &err)
     for (auto &ev : ev_) HIP_TRY(hipEventCreate(&ev));
     HIP_TRY(hipMalloc(&d_stat_, 16 * sizeof(double)));
     HIP_TRY(hipHostMalloc(&h_stat_, 16 * sizeof(double)));
+    // this device's copy of the no-alloc scratch array (with the device current)
+    HIP_TRY(hipGetSymbolAddress(&scratch_, HIP_SYMBOL(scratch)));
     return true;
 }
 
@@ -740,6 +750,84 @@ bool Device::finish(const Geometry &g, const Epilogue &ep, Real *d_field, int np
     return true;
 }
 
+template <class Real>
+bool Device::run_noalloc(const Geometry &g, int kernel, double scaling, const Epilogue &ep, Real *d_field,
+                         double *stats_out, std::string &err)
+{
+    HIP_TRY(hipSetDevice(device_));
+    const bool mfma = kernel == kKernelMfmaBf16x3 || kernel == kKernelMfmaF16x2 || kernel == kKernelMfmaBf16x2;
+    if (!g.structured || !mfma) {
+        err = "the allocation-free run takes the MFMA kernels of structured grids";
+        return false;
+    }
+    if (!d_harm_ || nharm_ <= 0) {
+        err = "no harmonic table resident on the device";
+        return false;
+    }
+    if (!scratch_) {
+        err = "the no-alloc scratch array has no device address";
+        return false;
+    }
+    const int np_ = nharm_;
+    // a tile chosen earlier for this grid, else the least-padding one: the
+    // tuner's timing runs would allocate
+    TileChoice tile;
+    if (!TileTuner::forced(tile) && TileTuner::enabled()) tuner().cached(g, kernel, sizeof(Real), tile);
+    MfmaPlan full;
+    if (!mfma_plan(g, np_, kernel, full, err, tile.nbx, tile.tk)) return false;
+    // scratch layout: statistics (16 doubles), points of interest, partials,
+    // then the operand tables of one chunk
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t stat_off = 0, poi_off = 256;
+    const size_t part_off = poi_off + up(ep.poi3.size() * sizeof(double));
+    const size_t tab_off = part_off + up(size_t(3) * full.nparts * sizeof(double));
+    if (tab_off >= kNoAllocScratch) {
+        err = "the grid's statistics partials and points of interest exceed the no-alloc scratch (" +
+              std::to_string(kNoAllocScratch >> 20) + " MiB)";
+        return false;
+    }
+    const size_t room = kNoAllocScratch - tab_off;
+    // the largest chunk (a multiple of 128 harmonics, at most the table) whose tables fit
+    int cap = static_cast<int>(round_up(np_, kFieldChunkHarmonics));
+    MfmaPlan plan;
+    for (;;) {
+        if (!mfma_plan(g, cap, kernel, plan, err, full.nbx, full.tk)) return false;
+        if (plan.table_bytes <= room) break;
+        const double per = double(plan.table_bytes) / cap;
+        const int next = static_cast<int>(room / per) / kFieldChunkHarmonics * kFieldChunkHarmonics;
+        cap = std::min(cap - kFieldChunkHarmonics, next);
+        if (cap < kFieldChunkHarmonics) {
+            err = "the grid's operand tables for 128 harmonics exceed the no-alloc scratch (" +
+                  std::to_string(kNoAllocScratch >> 20) + " MiB)";
+            return false;
+        }
+    }
+    last_nbx_ = plan.nbx;
+    last_tk_ = plan.tk;
+    plan.zscale = mfma_zscale(kernel, amp_max_);
+    char *base = static_cast<char *>(scratch_);
+    double *stat = reinterpret_cast<double *>(base + stat_off);
+    double *poi = reinterpret_cast<double *>(base + poi_off);
+    double *part = reinterpret_cast<double *>(base + part_off);
+    void *tab = base + tab_off;
+
+    timings_ = Timings{};
+    HIP_TRY(hipEventRecord(ev_[0], stream_));
+    // chunk by chunk: each chunk's tables overwrite the last one's (stream
+    // order: its field pass has read them) and the field carries the sums
+    for (int h0 = 0; h0 < np_; h0 += cap) {
+        const int nh = std::min(cap, np_ - h0);
+        plan.tab_s0 = h0 / 16;
+        if (!mfma_tables(plan, g, d_harm_, np_, tab, stream_, err, h0, nh)) return false;
+        if (h0 == 0) HIP_TRY(hipEventRecord(ev_[1], stream_));
+        const int flags = (h0 > 0 ? kFieldCarryIn : 0) | (h0 + nh == np_ ? kFieldFinal : 0);
+        if (!mfma_field(plan, g, tab, scaling, d_field, part, stream_, err, 0, 0, h0 / 16, nh / 16, flags))
+            return false;
+    }
+    const std::vector<long> none;
+    return finish<Real>(g, ep, d_field, plan.nparts, stats_out, none, none, err, nullptr, part, stat, poi);
+}
+
 std::vector<int> Device::chunk_targets(int nharm, int chunk)
 {
     std::vector<int> t;
@@ -883,6 +971,10 @@ template bool Device::run<float>(const Geometry &, int, double, const Epilogue &
                                  const SlabHook *);
 template bool Device::run<double>(const Geometry &, int, double, const Epilogue &, double *, double *,
                                   std::string &, const SlabHook *);
+template bool Device::run_noalloc<float>(const Geometry &, int, double, const Epilogue &, float *, double *,
+                                         std::string &);
+template bool Device::run_noalloc<double>(const Geometry &, int, double, const Epilogue &, double *, double *,
+                                          std::string &);
 template bool Device::run_streamed<float>(const Geometry &, int, int, double, const Epilogue &, float *, double *,
                                           std::string &, const HarmonicFeed &, bool *, const SlabHook *,
                                           int);
