@@ -8,6 +8,7 @@
 #include <condition_variable>
 #include <cstdlib>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -180,13 +181,49 @@ struct DrawContext {
 
     // the trial of src/scarflib_srm.f90:737-743 on one srng pair
     Real radial(Real r0) const { return r0 * (kmax - kmin) + kmin; }
-    bool accepts(Real r0, Real r1) const
+    Real acceptance(Real u) const
     {
-        const Real u = radial(r0);
-        const Real accept = static_cast<Real>(spec(static_cast<double>(u)) * static_cast<double>(cnst));
-        return r1 < accept;
+        return static_cast<Real>(spec(static_cast<double>(u)) * static_cast<double>(cnst));
     }
+    bool accepts(Real r0, Real r1) const { return r1 < acceptance(radial(r0)); }
     static Real uniform(int64_t raw) { return static_cast<Real>(Yarn2::to_uniform01(raw)); }
+
+    // Squeeze: bound[b] is at least the acceptance of every r0 in
+    // [b/kBins, (b+1)/kBins], so a trial with r1 >= bound[b] is rejected
+    // without evaluating the spectrum; the others take the exact test. The
+    // decisions are the exact test's. srm_fn * const = u^d psd(u) has one
+    // maximum, at u^2 = d / (2 nu - d) (von Karman) or 2d (Gaussian), with
+    // d = 2 in 3D and 1 in 2D; on a bin away from it the exact function is
+    // monotone, so its larger end bounds the bin, and the computed
+    // acceptance (a few roundings of `real` and one libm pow or exp, well
+    // under 1e-5 relative) stays below that end's value times 1 + 1e-4.
+    // Bins near the maximum have no bound.
+    static constexpr int kBins = 4096;
+    std::vector<Real> bound;
+    void make_bounds(int ndim)
+    {
+        const double d = ndim == 3 ? 2.0 : 1.0;
+        const double nu = static_cast<double>(spec.nu);
+        const double up2 = spec.acf == 0 ? (2.0 * nu > d ? d / (2.0 * nu - d) : -1.0) : 2.0 * d;
+        const double up = up2 > 0.0 ? std::sqrt(up2) : 0.0;
+        bound.assign(kBins, std::numeric_limits<Real>::infinity());
+        for (int b = 0; b < kBins; ++b) {
+            const Real ulo = radial(static_cast<Real>(b) / static_cast<Real>(kBins));
+            const Real uhi = radial(static_cast<Real>(b + 1) / static_cast<Real>(kBins));
+            Real end;
+            if (static_cast<double>(ulo) >= up * (1.0 + 1e-3)) end = acceptance(ulo);       // decreasing
+            else if (static_cast<double>(uhi) <= up * (1.0 - 1e-3)) end = acceptance(uhi);  // increasing
+            else continue;
+            if (std::isfinite(static_cast<double>(end)) && end >= Real(0))
+                bound[b] = static_cast<Real>(static_cast<double>(end) * (1.0 + 1e-4));
+        }
+    }
+    bool accepts_fast(Real r0, Real r1) const
+    {
+        const int b = std::min(static_cast<int>(r0 * static_cast<Real>(kBins)), kBins - 1);
+        if (!(r1 < bound[b])) return false;
+        return accepts(r0, r1);
+    }
 
     void fill_range(PairBlock<Real> &blk, long first, long last) const
     {
@@ -198,7 +235,7 @@ struct DrawContext {
             const size_t i = static_cast<size_t>(p - blk.start);
             blk.raw[2 * i] = a;
             blk.raw[2 * i + 1] = b;
-            blk.accept[i] = accepts(uniform(a), uniform(b)) ? 1 : 0;
+            blk.accept[i] = accepts_fast(uniform(a), uniform(b)) ? 1 : 0;
         }
     }
 };
@@ -263,6 +300,7 @@ public:
         ctx_.kc = static_cast<Real>(s.kc);
         ctx_.cnst = static_cast<Real>(s.cnst);
         ctx_.seed = static_cast<unsigned long>(seed);  // set_seed: unsigned long myseed = seed
+        ctx_.make_bounds(s.ndim);
         cl_[0] = static_cast<Real>(cfg.cl[0]);
         cl_[1] = static_cast<Real>(cfg.cl[1]);
         cl_[2] = static_cast<Real>(cfg.cl[2]);

@@ -38,8 +38,19 @@ public:
         const uint64_t t = static_cast<uint64_t>(kA0) * static_cast<uint64_t>(r0_) +
                            static_cast<uint64_t>(kA1) * static_cast<uint64_t>(r1_);
         r1_ = r0_;
-        r0_ = static_cast<int64_t>(t % static_cast<uint64_t>(kModulus));
+        r0_ = static_cast<int64_t>(mod_m(t));
         return r0_ == 0 ? 0 : pow_generator(r0_);
+    }
+
+    // x mod m for x < 2^63, with m = 2^31 - 1 a Mersenne prime: since
+    // 2^31 = 1 (mod m), x = (x >> 31) * 2^31 + (x & m) = (x >> 31) + (x & m);
+    // two folds leave a value <= m + 3, one conditional subtraction finishes.
+    // Exact, like %, without a division.
+    static uint64_t mod_m(uint64_t x)
+    {
+        x = (x & kM) + (x >> 31);
+        x = (x & kM) + (x >> 31);
+        return x >= kM ? x - kM : x;
     }
 
     // uniform01_dist<double>: one draw scaled into [0, 1)
@@ -107,7 +118,7 @@ private:
     {
         const PowTables &t = tables();
         const uint64_t u = static_cast<uint64_t>(e);
-        return static_cast<int64_t>(static_cast<uint64_t>(t.hi[u >> 16]) * t.lo[u & 0xffffu] % kM);
+        return static_cast<int64_t>(mod_m(static_cast<uint64_t>(t.hi[u >> 16]) * t.lo[u & 0xffffu]));
     }
 
     int64_t r0_ = 0;

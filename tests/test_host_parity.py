@@ -121,6 +121,36 @@ def test_resumable_draw_is_cut_invariant(oracle, lib_f, lib_d, flavour, chunk):
     assert n1 == n2 and np.array_equal(u1, u2) and np.array_equal(h1, h2)
 
 
+SQUEEZE_CASES = [
+    # the band [kmin, kmax] spans the maximum of srm_fn * const: bins on both
+    # sides of it take the bound, the bins around it the exact test
+    dict(n=(64, 64, 64), ds=20.0, hurst=0.25, cl=(200.0,) * 3, nharm=900),                 # VK 3D, peak 1.15
+    dict(n=(80, 80, 1), ds=20.0, hurst=0.4, cl=(150.0, 150.0, 0.0), nharm=900),            # VK 2D, peak 0.77
+    dict(n=(40, 40, 40), ds=50.0, acf=1, cl=(300.0,) * 3, nharm=900),                     # Gaussian 3D, peak 2
+    dict(n=(60, 60, 1), ds=25.0, acf=1, cl=(200.0, 200.0, 0.0), nharm=900),               # Gaussian 2D, peak 1.41
+]
+
+
+@pytest.mark.parametrize("flavour", ["f", "d"])
+@pytest.mark.parametrize("case", range(len(SQUEEZE_CASES)))
+def test_squeezed_trials_decide_as_the_exact_test(oracle, lib_f, lib_d, flavour, case):
+    """The draw rejects most trials against a per-bin upper bound of the
+    acceptance and evaluates the spectrum only for the rest; with the
+    spectrum's maximum inside the sampled band (bins rising, falling, and the
+    unbounded bins around the maximum), the draw and the uniforms consumed
+    equal the oracle's, which evaluates every trial."""
+    lib = lib_f if flavour == "f" else lib_d
+    real = np.float32 if flavour == "f" else np.float64
+    cfg = structured_cfg(real=real, **SQUEEZE_CASES[case])
+    p = oracle.params(cfg)
+    s = oracle.setup(p, flavour)
+    init_product(lib, cfg)
+    for seed in (3, 4444):
+        h1, u1, n1 = oracle.draw(p, s, seed, flavour)
+        h2, u2, n2 = lib.draw_harmonics(seed, cfg["nharm"])
+        assert n1 == n2 and np.array_equal(u1, u2) and np.array_equal(h1, h2)
+
+
 def test_stream_chunk_option_checked(lib_f):
     """Streamed chunks start at multiples of the field kernels' 128-harmonic
     promotion period: other chunk sizes are refused."""
