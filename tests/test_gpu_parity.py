@@ -225,9 +225,12 @@ def test_every_forced_tile_gives_the_same_field(lib_f, monkeypatch, kernel):
 
 
 def test_host_copy_equals_device_field(lib_f):
-    """scarf_execute stages fields above 32 MiB through pinned buffers; the
-    host array equals the device field bit for bit (72 MB: two whole 32 MiB
-    chunks and a partial one)."""
+    """scarf_execute returns a 72 MB field (above the 64 MiB slab threshold)
+    in z slabs of 11.5 MB, each staged through the two pinned buffers in
+    chunks of 4 MiB (two whole chunks and a partial one per slab: the buffer
+    reuse, the partial last chunk and the pooled drain all run), the last,
+    2.9 MB slab by one copy; the host array equals the device field bit for
+    bit."""
     from scarf3d_amd.hip import Hip
 
     hip = Hip()
