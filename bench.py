@@ -453,9 +453,16 @@ def main(argv=None) -> int:
     if "direct_f32" in res and kernel != "direct_f32":
         out["speedup_vs_fp32_direct"] = _sig(res["direct_f32"]["field_ms"] / main_r["field_ms"])
     for k, r in out["kernels"].items():
-        r.setdefault("gpt_harm_per_s", _sig(work / (r["field_ms"] * 1e-3) / 1e9))
+        rate = work / (r["field_ms"] * 1e-3)  # point-harmonics per second
+        r.setdefault("gpt_harm_per_s", _sig(rate / 1e9))
         if "direct_f32" in res and k != "direct_f32":
             r["speedup_vs_fp32_direct"] = _sig(res["direct_f32"]["field_ms"] / r["field_ms"])
+        # the rooflines of north_star: the reference formulation's 10 FP32 flops per
+        # point-harmonic against the FP32 vector peak, and the executed MFMA flops
+        # (4 per product per point-harmonic) against dense bf16/fp16
+        r["frac_fp32_valu"] = _sig(rate * 10 / (PEAK_VALU_TFLOPS * 1e12))
+        if k in MFMA_PRODUCTS:
+            r["frac_bf16_mfma"] = _sig(rate * 4 * MFMA_PRODUCTS[k] / (PEAK_MFMA_TFLOPS * 1e12))
     out.update(extra)
     if e2e:
         # the reference's call, scarf_execute, warm: draw + device + copy into the host array
@@ -472,10 +479,13 @@ def main(argv=None) -> int:
     lib.finalize()
     if args.workload == "precision_512" and not args.no_ensemble:
         e = measure_ensemble("mfma_bf16x3", max(3, min(args.steps, 10)), 2)
+        single = out["kernels"].get("mfma_bf16x3", {}).get("frac_bf16_mfma")
         out["ensemble_64"] = {
             "kernel": "mfma_bf16x3", "members": e["members"], "point_harmonics": e["work"], "ms": _sig(e["ms"]),
             "gpt_harm_per_s": _sig(e["work"] / (e["ms"] * 1e-3) / 1e9), "achieved_tflops": _sig(e["achieved"]),
-            "frac": _sig(e["frac"]), "first_run_s": _sig(e["first_s"]), "ms_per_call_wall": _sig(e["wall_ms"]),
+            "frac": _sig(e["frac"]), "single_field_frac": single,
+            "gap_to_single_field": _sig(1.0 - e["frac"] / single) if single else None,
+            "first_run_s": _sig(e["first_s"]), "ms_per_call_wall": _sig(e["wall_ms"]),
             "note": "one persistent grouped-GEMM launch per realization set, replayed from a hipGraph; ms = device "
                     "time of the replay (HIP events), frac against 2.5 PFLOP/s dense bf16 at 24 flop per "
                     "point-harmonic"}
