@@ -35,12 +35,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
-from scarf3d_amd.workloads import WORKLOADS, ensemble_members, init_library, workload_cfg  # noqa: E402
+from scarf3d_amd.workloads import WORKLOADS, ensemble_members, init_library, structured_cfg, workload_cfg  # noqa: E402
 
 METRIC = "SRM Gpt·harm/s at 512³×20k: split-BF16 MFMA vs FP32 direct, err≤1e-5σ"
 PEAK_MFMA_TFLOPS = 2500.0  # MI355X dense bf16 / fp16
 PEAK_VALU_TFLOPS = 157.3   # MI355X FP32 vector
 PEAK_VALU_F64_TFLOPS = 78.6  # MI355X FP64 vector
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 MFMA_PRODUCTS = {"mfma_bf16x3": 6, "mfma_f16x2": 3, "mfma_bf16x2": 3}
 
 KERNELS = ("mfma_bf16x3", "mfma_f16x2", "mfma_bf16x2", "direct_f32", "direct_f64")
@@ -212,6 +213,97 @@ def measure_end_to_end(lib, w, kernel, steps, warmup):
     return out
 
 
+# FIM at the reference examples' configuration (examples/c/driver.c:76-96,
+# :405-430: 500^3, ds = dh = 50 m, von Karman cl = 2000 m, Hurst 0.25,
+# sigma 0.05, seed 1235, no padding): internal grid 502^3
+FIM_CASE = dict(n=(500, 500, 500), ds=50.0, acf=0, hurst=0.25, cl=(2000.0,) * 3, sigma=0.05, seed=1235)
+
+
+def fim_bytes(npts, nout):
+    """Algorithmic HBM bytes of one FIM realization (float library): the half
+    spectrum written once (8 B per complex), read and written once by the
+    (y, z) transforms (16 B) and read by the x transform (8 B), the internal
+    field written by it (4 B), read and written by the scaling (8 B), and the
+    external field written by the mapping with one internal value read per
+    output point (8 B)."""
+    n1, n2, n3 = npts
+    half = (n1 // 2 + 1) * n2 * n3
+    return 8 * half + 16 * half + 8 * half + 4 * n1 * n2 * n3 + 8 * n1 * n2 * n3 + 8 * nout
+
+
+def measure_fim(steps, warmup, cpu=True):
+    """FIM (method 0) at FIM_CASE: the first call of the process (runtime,
+    transform plans, buffers), then warm scarf_execute_device (field in HBM)
+    and scarf_execute (field into a host array) calls, wall clock, with the
+    device pass times the library reports in stats(5:8)."""
+    import scarf3d_amd
+    from scarf3d_amd.hip import Hip
+
+    lib = scarf3d_amd.load("f")
+    c = FIM_CASE
+    cfg = structured_cfg(n=c["n"], ds=c["ds"], acf=c["acf"], hurst=c["hurst"], cl=c["cl"], sigma=c["sigma"],
+                         real=np.float32)
+    cfg["method"] = 0
+    init_library(lib, cfg)
+    hip = Hip()
+    d = hip.empty(lib.npts, np.float32)
+    t0 = time.perf_counter()
+    st = lib.execute_device(c["seed"], d.ptr)
+    first_ms = (time.perf_counter() - t0) * 1e3
+    if lib.last_error():
+        raise RuntimeError(lib.last_error())
+    for _ in range(warmup):
+        lib.execute_device(c["seed"], d.ptr)
+    ms, parts = [], []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        st = lib.execute_device(c["seed"], d.ptr)
+        ms.append((time.perf_counter() - t1) * 1e3)
+        parts.append([float(v) * 1e3 for v in st[4:8]])
+    host = np.zeros(lib.npts, dtype=np.float32)
+    hms = []
+    for _ in range(max(2, steps // 2)):
+        t1 = time.perf_counter()
+        lib.execute(c["seed"], out=host)
+        hms.append((time.perf_counter() - t1) * 1e3)
+    d.free()
+    lib.finalize()
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "oracle"))
+    import fim_oracle as fo
+
+    s = fo.setup(dict(cfg, pad=0), np.float32)
+    npts = s["npts"]
+    nint_ = int(np.prod(npts))
+    nout = int(np.prod(c["n"]))
+    dev = float(np.mean(ms))
+    part = np.mean(np.array(parts), axis=0)
+    B = fim_bytes(npts, nout)
+    out = {"workload": "fim_500", "internal_grid": npts, "first_call_ms": _sig(first_ms), "ms": _sig(dev),
+           "min_ms": _sig(float(np.min(ms))), "execute_host_ms": _sig(float(np.mean(hms))),
+           "spectrum_ms": _sig(part[0]), "symmetry_ms": _sig(part[1]), "transforms_ms": _sig(part[2]),
+           "mapping_ms": _sig(part[3]), "mpts_per_s": _sig(nout / (dev * 1e-3) / 1e6),
+           "internal_mpts_per_s": _sig(nint_ / (dev * 1e-3) / 1e6),
+           "hbm": {"algorithmic_bytes": B, "achieved_gbs": _sig(B / (dev * 1e-3) / 1e9), "peak_gbs": PEAK_HBM_GBS,
+                   "frac": _sig(B / (dev * 1e-3) / 1e9 / PEAK_HBM_GBS)},
+           "note": "scarf_execute_device wall clock per call (spectrum, symmetry, hipFFT (y, z) c2c + x c2r, "
+                   "statistics, mapping); stage ms from the library's stats(5:8), HIP events"}
+    log(f"[bench] FIM {c['n']} (internal {npts}): first call {first_ms:.1f} ms, {dev:.2f} ms per call "
+        f"(spectrum {part[0]:.2f}, symmetry {part[1]:.3f}, transforms {part[2]:.2f}, mapping {part[3]:.2f} ms), "
+        f"host-array call {float(np.mean(hms)):.1f} ms")
+    if cpu:
+        # the FP64 oracle on one core on a bounded sample (a 160^3 block of the same model parameters)
+        cs = structured_cfg(n=(160, 160, 160), ds=c["ds"], acf=c["acf"], hurst=c["hurst"], cl=c["cl"],
+                            sigma=c["sigma"], real=np.float32)
+        t1 = time.perf_counter()
+        r = fo.realization(dict(cs, method=0, pad=0, structured=True), c["seed"], np.float32)
+        secs = time.perf_counter() - t1
+        ni = int(np.prod(r["setup"]["npts"]))
+        out["cpu_baseline"] = {"value": _sig(ni / secs / 1e6), "unit": "internal Mpt/s", "cores": 1, "kind": "port",
+                               "sample": f"160^3 grid (internal {r['setup']['npts']}), FP64 numpy restatement "
+                                         f"({secs:.1f} s)"}
+    return out
+
+
 def _host_threads():
     """Every core this process may run on (its affinity set)."""
     return len(os.sched_getaffinity(0))
@@ -299,12 +391,13 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS) + ["ensemble_64", "copy_calib"])
+    ap.add_argument("--workload", default="precision_512", choices=sorted(WORKLOADS) + ["ensemble_64", "copy_calib", "fim_500"])
     ap.add_argument("--kernel", default=None, choices=KERNELS, help="timed kernel (default: the workload's)")
     ap.add_argument("--no-compare", action="store_true", help="skip the comparison kernels and the FP64 reference")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end scarf_execute timings")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble (default workload)")
+    ap.add_argument("--no-fim", action="store_true", help="skip the FIM (method 0) timing (default workload)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SRM shards by z-slab ranges with no exchange inside the harmonic "
@@ -316,6 +409,17 @@ def main(argv=None) -> int:
         return bench_ensemble(args)
     if args.workload == "copy_calib":
         return bench_copy_calibration(args)
+    if args.workload == "fim_500":
+        f = measure_fim(args.steps, args.warmup, cpu=not args.no_cpu)
+        out = {"metric": "FIM realizations: 500^3 grid (reference examples), external Mpt/s", "value": f["mpts_per_s"],
+               "unit": "Mpt/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": f["ms"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+               "data": "synthetic", "config": {k: v for k, v in f.items() if k not in ("hbm", "cpu_baseline")},
+               "roofline": {"bound": "hbm", "achieved": f["hbm"]["achieved_gbs"], "peak": PEAK_HBM_GBS,
+                            "unit": "GB/s", "frac": f["hbm"]["frac"], "traffic": None, "kernel": "whole FIM pass"},
+               "cpu_baseline": f.get("cpu_baseline")}
+        print(json.dumps(out), flush=True)
+        return 0
 
     import scarf3d_amd
     from oracle_lib import Oracle, epilogue_at
@@ -489,6 +593,8 @@ def main(argv=None) -> int:
             "note": "one persistent grouped-GEMM launch per realization set, replayed from a hipGraph; ms = device "
                     "time of the replay (HIP events), frac against 2.5 PFLOP/s dense bf16 at 24 flop per "
                     "point-harmonic"}
+    if args.workload == "precision_512" and not args.no_fim:
+        out["fim_500"] = measure_fim(max(3, min(args.steps, 10)), 1, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
     return 0
 

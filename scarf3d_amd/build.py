@@ -38,6 +38,8 @@ UNITS = [
     ("srm_mfma.hip", DEVICE, "srm_mfma.o"),
     ("srm_host.cpp", DEVICE + HOST_EXACT, "srm_host.o"),
     ("quadpack.cpp", DEVICE + HOST_EXACT, "quadpack.o"),
+    ("fim.hip", DEVICE, "fim.o"),
+    ("fim_host.cpp", DEVICE + HOST_EXACT, "fim_host.o"),
     ("scarf_capi.cpp", DEVICE + HOST_EXACT, "scarf_capi_f.o"),
     ("scarf_capi.cpp", DEVICE + HOST_EXACT + ["-DDOUBLE_PREC"], "scarf_capi_d.o"),
 ]
@@ -57,7 +59,8 @@ def _compile(src: str, flags: list[str], obj: str, verbose: bool) -> None:
 
 
 def _link(objs: list[str], out: Path, verbose: bool) -> None:
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(out), *[str(BUILD / o) for o in objs]]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(out), *[str(BUILD / o) for o in objs],
+           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -154,7 +157,7 @@ def build(jobs: int = 4, verbose: bool = False) -> list[Path]:
         futs = [ex.submit(_compile, s, f, o, verbose) for s, f, o in UNITS]
         for fut in futs:
             fut.result()
-    shared = ["srm_kernels.o", "srm_mfma.o", "srm_host.o", "quadpack.o"]
+    shared = ["srm_kernels.o", "srm_mfma.o", "srm_host.o", "quadpack.o", "fim.o", "fim_host.o"]
     outs = []
     for obj, name in (("scarf_capi_f.o", "libscarf3df.so"), ("scarf_capi_d.o", "libscarf3d.so")):
         out = LIB / name

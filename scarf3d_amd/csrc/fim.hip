@@ -1,0 +1,552 @@
+// FIM device passes and their launch sequence (see fim.hpp).
+//
+// Layout in HBM, one realization:
+//   S  half spectrum, complex `real`, H x N2 x N3 with H = N1/2 + 1, index
+//      i + H (j + N2 k) (kx fastest): the only part of the reference's full
+//      spectrum that its c2r transform along x reads
+//   R  internal field, `real`, N1 x N2 x N3, index i + N1 (j + N2 k)
+//   partials (3 doubles per block), statistics (6 doubles), points of
+//   interest; the generator tables (fim_stream.hpp: 18 KB of powers of the
+//   primitive root, the companion-matrix powers), uploaded once per process.
+// Work per internal point (bytes of HBM): spectrum 8 (written), symmetry
+// reads/writes 2 planes, transforms ~3 passes over S plus R, statistics 4+4
+// (R read, scaled R written), epilogue 8 if active, mapping 4 per output
+// point plus the gathered R values.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "fim.hpp"
+#include "srm_stats.hpp"
+#include "fim_stream.hpp"
+
+namespace scarf {
+namespace {
+
+using fim_stream::kJumpBits;
+using fim_stream::kPowWords;
+using GenTables = fim_stream::Tables;
+
+template <class Real>
+struct Cplx {
+    Real re, im;
+};
+
+// Spectrum parameters (compute_spectrum, src/scarflib_fim.f90:2440-2510)
+template <class Real>
+struct SpecArgs {
+    int n1, n2, n3, h;      // internal grid, h = n1/2 + 1
+    long lines;             // n2 * n3 x-lines
+    int lines_per_item;
+    uint32_t s1[2];         // generator state after its first draw (state of stream position 0)
+    uint32_t line_step[4];  // A^n1
+    Real dk[3], cl[3], kc, nu, cnst;
+    int filter, acf;
+};
+
+// Half spectrum with the random phases of the global stream: internal
+// point (i, j, k) (0-based) takes the uniform at stream position
+// i + n1 (j + n2 k), the Fortran-order index that the reference's prng gives
+// every point of the grid whatever the process split
+// (src/prng.cpp:66-130: seed, then jumps over the points other processes own).
+// One wave per item of lines_per_item consecutive x-lines: the wave jumps to
+// its first line's start (companion-matrix powers A^(2^b)), lane l steps to
+// position l, then every lane advances by A^64 per point and every line by
+// A^n1. Stores are coalesced (lane l writes point i = l + 64 t).
+template <class Real>
+__global__ __launch_bounds__(256) void fim_spectrum(Cplx<Real> *__restrict__ S, SpecArgs<Real> a,
+                                                    const GenTables *__restrict__ gt, long items)
+{
+#pragma clang fp contract(off)
+    __shared__ uint32_t pw[kPowWords];
+    for (int q = threadIdx.x; q < kPowWords; q += blockDim.x) pw[q] = gt->pw[q];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const long waves = long(gridDim.x) * (blockDim.x >> 6);
+    const Real pi = static_cast<Real>(3.141592653589793);
+    for (long item = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); item < items; item += waves) {
+        const long l0 = item * a.lines_per_item;
+        const long l1 = min(l0 + a.lines_per_item, a.lines);
+        fim_stream::walk_item(pw, gt, a.s1, a.line_step, a.n1, a.h, l0, l1, lane, [&](long line, int i, uint32_t raw) {
+#pragma clang fp contract(off)
+            const int j = int(line % a.n2), k = int(line / a.n2);
+            const Real ky = static_cast<Real>(j <= a.n2 / 2 ? j : a.n2 - j) * a.dk[1];
+            const Real kz = static_cast<Real>(k <= a.n3 / 2 ? k : a.n3 - k) * a.dk[2];
+            const Real u = static_cast<Real>(static_cast<double>(raw) / 2147483647.0);
+            const Real kx = static_cast<Real>(i) * a.dk[0];
+            const Real kr = sqrt(kx * kx + ky * ky + kz * kz);
+            const Real butter = (a.filter && kr > a.kc) ? Real(0) : Real(1);
+            const Real x1 = kx * a.cl[0], x2 = ky * a.cl[1], x3 = kz * a.cl[2];
+            const Real kl = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+            const Real fn = a.acf == 0 ? Real(1) / pow(Real(1) + kl * kl, a.nu) : exp(Real(-0.25) * (kl * kl));
+            const Real amp = sqrt(a.cnst * fn) * butter;
+            const Real ph = (u * Real(2)) * pi;
+            const Real c = cos(ph), sn = sin(ph);
+            S[size_t(i) + size_t(a.h) * size_t(line)] = Cplx<Real>{c * amp, sn * amp};
+        });
+    }
+}
+
+// Hermitian conditions on the kx = 0 and kx = Nyquist planes as one process
+// holding the whole grid applies them (enforce_symmetry,
+// src/scarflib_fim.f90:2660-3008), 0-based with jn = n2/2, kn = n3/2:
+//   point conditions, in order: S(0,0,0) = 0 (plane 0); S(N,0,0) real (plane
+//   N = n1/2); S(p,jn,kn), S(p,0,kn), S(p,jn,0) real;
+//   then, from the values after the point conditions (B):
+//   k > kn, every j:        S(p,j,k)  = conj(B(p, -j mod n2, n3 - k))
+//   k = kn, j > jn:         S(p,j,kn) = conj(B(p, n2 - j, kn))
+//   k = 0,  j > jn:         S(p,j,0)  = conj(B(p, n2 - j, 0)) in 2D, and
+//                           conj(B(p, j, 0)) in 3D: with one process the
+//                           reference's exchange of that line (its "beta"
+//                           region) is replaced by the exchange of the j = 0
+//                           column, whose subarray types overwrite it
+//                           (:2866-2960), so the line is conjugated in place.
+// Every position is written by at most one rule and every value read comes
+// from a position no rule writes, so one pass in place is exact.
+template <class Real>
+__global__ __launch_bounds__(256) void fim_symmetry(Cplx<Real> *__restrict__ S, int n1, int n2, int n3, int h)
+{
+    const long per_plane = long(n2) * n3;
+    const long e = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (e >= 2 * per_plane) return;
+    const int p = e < per_plane ? 0 : n1 / 2;
+    const long r = e % per_plane;
+    const int j = int(r % n2), k = int(r / n2);
+    const int jn = n2 / 2, kn = n3 / 2;
+    auto at = [&](int jj, int kk) -> Cplx<Real> & { return S[size_t(p) + size_t(h) * (size_t(jj) + size_t(n2) * kk)]; };
+    Cplx<Real> &v = at(j, k);
+    const bool point = (j == 0 && k == 0) || (j == jn && k == kn) || (j == 0 && k == kn) || (j == jn && k == 0);
+    if (point) {
+        if (p == 0 && j == 0 && k == 0) v = Cplx<Real>{Real(0), Real(0)};
+        if (p == n1 / 2 && j == 0 && k == 0) v.im = Real(0);
+        if (j == jn && k == kn) v.im = Real(0);
+        if (j == 0 && k == kn) v.im = Real(0);
+        if (j == jn && k == 0) v.im = Real(0);
+        return;
+    }
+    const int jp = j == 0 ? 0 : n2 - j;
+    if (k > kn) {
+        const Cplx<Real> b = at(jp, n3 - k);
+        v = Cplx<Real>{b.re, -b.im};
+    } else if (j > jn && k == kn) {
+        const Cplx<Real> b = at(n2 - j, kn);
+        v = Cplx<Real>{b.re, -b.im};
+    } else if (j > jn && k == 0) {
+        const Cplx<Real> b = n3 == 1 ? at(n2 - j, 0) : v;
+        v = Cplx<Real>{b.re, -b.im};
+    }
+}
+
+// After the (y, z) transforms: the c2r transform along x reads only the real
+// parts of the kx = 0 and Nyquist coefficients of every x-line (the
+// half-complex form of FFTW's c2r has no slot for their imaginary parts), so
+// they are set to zero explicitly and any c2r implementation agrees.
+template <class Real>
+__global__ __launch_bounds__(256) void fim_dc_real(Cplx<Real> *__restrict__ S, int n1, int h, long lines)
+{
+    const long e = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (e >= 2 * lines) return;
+    const long line = e >> 1;
+    const int i = (e & 1) ? n1 / 2 : 0;
+    S[size_t(i) + size_t(h) * size_t(line)].im = Real(0);
+}
+
+// buffer = REAL(spec) * scaling (src/scarflib_fim.f90:955-961) and the
+// block moments of the scaled values, FP64 (count, mean, M2 per block)
+template <class Real>
+__global__ __launch_bounds__(256) void fim_scale(Real *__restrict__ R, long n, Real scaling, double *__restrict__ part)
+{
+    Moments m{0.0, 0.0, 0.0};
+    const long stride = long(gridDim.x) * blockDim.x;
+    for (long base = blockIdx.x * long(blockDim.x) + threadIdx.x; base < n; base += 8 * stride) {
+        double v[8];
+        bool ok[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long e = base + q * stride;
+            ok[q] = e < n;
+            v[q] = 0.0;
+            if (ok[q]) {
+                const Real x = R[e] * scaling;
+                R[e] = x;
+                v[q] = static_cast<double>(x);
+            }
+        }
+        merge_moments(m, local_moments(v, ok));
+    }
+    block_moments(m, part + 3 * blockIdx.x);
+}
+
+// mean, variance, info(3) = sqrt(real(var)), info(4) = real(mean) and the
+// rescale factor sigma / info(3) (src/scarflib_fim.f90:1001-1031; one
+// process: parallel_variance returns its single set unchanged)
+template <class Real>
+__global__ __launch_bounds__(256) void fim_finalize(const double *__restrict__ part, int nparts, double sigma,
+                                                    double *__restrict__ stat)
+{
+    __shared__ double sm[3][256];
+    Moments m{0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += 256) merge_moments(m, Moments{part[3 * b], part[3 * b + 1], part[3 * b + 2]});
+    sm[0][threadIdx.x] = m.n;
+    sm[1][threadIdx.x] = m.mean;
+    sm[2][threadIdx.x] = m.m2;
+    __syncthreads();
+    for (int stride = 128; stride >= 1; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            Moments x{sm[0][threadIdx.x], sm[1][threadIdx.x], sm[2][threadIdx.x]};
+            merge_moments(x, Moments{sm[0][threadIdx.x + stride], sm[1][threadIdx.x + stride], sm[2][threadIdx.x + stride]});
+            sm[0][threadIdx.x] = x.n;
+            sm[1][threadIdx.x] = x.mean;
+            sm[2][threadIdx.x] = x.m2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = sm[0][0], mean = sm[1][0], m2 = sm[2][0];
+        const double var = n > 1.0 ? m2 / (n - 1.0) : 0.0;
+        const Real info3 = sqrt(static_cast<Real>(var));
+        const Real info4 = static_cast<Real>(mean);
+        stat[0] = n;
+        stat[1] = mean;
+        stat[2] = var;
+        stat[3] = info3;
+        stat[4] = info4;
+        stat[5] = static_cast<Real>(sigma) / info3;
+    }
+}
+
+// rescale (buffer * (sigma / info(3)) - info(4)) and the mute/taper of every
+// point of interest on the internal grid, positions (i - 1) dh
+// (src/scarflib_fim.f90:1014-1059, tapering_structured with ls = 1)
+template <class Real>
+__global__ __launch_bounds__(256) void fim_epilogue(Real *__restrict__ R, int n1, int n2, int n3, Real dh,
+                                                    const double *__restrict__ stat, int rescale,
+                                                    const Real *__restrict__ pt, int npoi, Real mute, Real taper)
+{
+#pragma clang fp contract(off)
+    const long n = long(n1) * n2 * n3;
+    const Real sc = static_cast<Real>(stat[5]), mu = static_cast<Real>(stat[4]);
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
+        Real v = R[e];
+        if (rescale) v = v * sc - mu;
+        if (npoi > 0) {
+            const int i = int(e % n1), j = int((e / n1) % n2), k = int(e / (long(n1) * n2));
+            for (int q = 0; q < npoi; ++q) {
+                const Real dx = static_cast<Real>(i) * dh - pt[3 * q];
+                const Real dy = static_cast<Real>(j) * dh - pt[3 * q + 1];
+                const Real dz = static_cast<Real>(k) * dh - pt[3 * q + 2];
+                v = taper_factor(v, dx * dx, dy * dy, dz * dz, mute, taper);
+            }
+        }
+        R[e] = v;
+    }
+}
+
+// Nearest internal value of a point at internal coordinates (x, y, z)
+// relative to off_axis (order_points_* and interpolate,
+// src/scarflib_fim.f90:1514-1704, :3086-3120, one process, identity
+// rotation): c = (x / dh + 3) - 1 along each axis, inside when
+// 1 <= c < npts + 1, value delta(NINT(c) - 1) of the halo-extended field
+// (index 0 is the halo: zero along x and y, the last z-plane along z, which
+// is periodic in the reference's process topology, :110, :3013-3080);
+// outside points stay 0.
+template <class Real>
+__device__ inline Real nearest(const Real *__restrict__ R, int n1, int n2, int n3, Real dh, Real x, Real y, Real z)
+{
+#pragma clang fp contract(off)
+    const Real c[3] = {(x / dh + Real(3)) - Real(1), (y / dh + Real(3)) - Real(1), (z / dh + Real(3)) - Real(1)};
+    const int np[3] = {n1, n2, n3};
+    long idx[3];
+    for (int d = 0; d < 3; ++d) {
+        if (!(c[d] >= Real(1) && c[d] < static_cast<Real>(np[d] + 1))) return Real(0);
+        idx[d] = lround(c[d]) - 1;  // 0 (halo) .. npts
+    }
+    if (idx[0] == 0 || idx[1] == 0) return Real(0);
+    if (idx[2] == 0) idx[2] = n3;
+    return R[size_t(idx[0] - 1) + size_t(n1) * (size_t(idx[1] - 1) + size_t(n2) * size_t(idx[2] - 1))];
+}
+
+// structured block fs..fe of step ds: external point (fs - 1 + i0) ds
+// (grid_mapping_struct / order_points_struct, src/scarflib_fim.f90:1303-1512, :1598-1700)
+template <class Real>
+__global__ __launch_bounds__(256) void fim_map_grid(const Real *__restrict__ R, int n1, int n2, int n3, Real dh,
+                                                    Real *__restrict__ out, int nx, int ny, int nz, int fx, int fy,
+                                                    int fz, Real ds, Real ox, Real oy, Real oz)
+{
+#pragma clang fp contract(off)
+    const long n = long(nx) * ny * nz;
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
+        const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (long(nx) * ny));
+        const Real x = static_cast<Real>(fx - 1 + i) * ds - ox;
+        const Real y = static_cast<Real>(fy - 1 + j) * ds - oy;
+        const Real z = static_cast<Real>(fz - 1 + k) * ds - oz;
+        out[e] = nearest(R, n1, n2, n3, dh, x, y, z);
+    }
+}
+
+// point list (FP64 copies of the caller's `real` coordinates)
+// (grid_mapping_unstruct / order_points_unstruct, src/scarflib_fim.f90:1110-1300, :1514-1596)
+template <class Real>
+__global__ __launch_bounds__(256) void fim_map_points(const Real *__restrict__ R, int n1, int n2, int n3, Real dh,
+                                                      Real *__restrict__ out, const double *__restrict__ px,
+                                                      const double *__restrict__ py, const double *__restrict__ pz,
+                                                      long n, Real ox, Real oy, Real oz)
+{
+#pragma clang fp contract(off)
+    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
+        const Real x = static_cast<Real>(px[e]) - ox;
+        const Real y = static_cast<Real>(py[e]) - oy;
+        const Real z = static_cast<Real>(pz[e]) - oz;
+        out[e] = nearest(R, n1, n2, n3, dh, x, y, z);
+    }
+}
+
+template <class Real>
+struct FftTypes;
+template <>
+struct FftTypes<float> {
+    static constexpr hipfftType c2c = HIPFFT_C2C, c2r = HIPFFT_C2R;
+    static hipfftResult exec_c2c(hipfftHandle p, void *d) { return hipfftExecC2C(p, (hipfftComplex *)d, (hipfftComplex *)d, HIPFFT_BACKWARD); }
+    static hipfftResult exec_c2r(hipfftHandle p, void *in, void *out) { return hipfftExecC2R(p, (hipfftComplex *)in, (hipfftReal *)out); }
+};
+template <>
+struct FftTypes<double> {
+    static constexpr hipfftType c2c = HIPFFT_Z2Z, c2r = HIPFFT_Z2D;
+    static hipfftResult exec_c2c(hipfftHandle p, void *d) { return hipfftExecZ2Z(p, (hipfftDoubleComplex *)d, (hipfftDoubleComplex *)d, HIPFFT_BACKWARD); }
+    static hipfftResult exec_c2r(hipfftHandle p, void *in, void *out) { return hipfftExecZ2D(p, (hipfftDoubleComplex *)in, (hipfftDoubleReal *)out); }
+};
+
+constexpr int kScaleBlocks = 1024;
+
+}  // namespace
+
+struct FimDevice::Impl {
+    GenTables *d_gen = nullptr;
+    void *d_spec = nullptr;
+    size_t spec_cap = 0;
+    void *d_real = nullptr;
+    size_t real_cap = 0;
+    double *d_part = nullptr, *d_stat = nullptr, *h_stat = nullptr;
+    void *d_pt = nullptr;
+    size_t pt_cap = 0;
+    hipfftHandle plan_yz = 0, plan_x = 0;
+    bool plans = false;
+    int plan_n[3] = {0, 0, 0};
+    int plan_real = 0;  // sizeof(Real) of the plans
+    hipEvent_t ev[6] = {};
+    size_t reserved = 0;
+
+    bool grow(void **p, size_t &cap, size_t bytes, std::string &err)
+    {
+        if (bytes <= cap) return true;
+        if (*p) {
+            (void)hipFree(*p);
+            reserved -= cap;
+            *p = nullptr;
+            cap = 0;
+        }
+        const hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            err = std::string("FIM: hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e);
+            return false;
+        }
+        cap = bytes;
+        reserved += bytes;
+        return true;
+    }
+    void drop_plans()
+    {
+        if (!plans) return;
+        (void)hipfftDestroy(plan_yz);
+        (void)hipfftDestroy(plan_x);
+        plans = false;
+    }
+};
+
+FimDevice::FimDevice() : impl_(new Impl) {}
+
+FimDevice::~FimDevice()
+{
+    // like the SRM device buffers, released by the process exit (the HIP
+    // runtime may already be tearing down when static objects are destroyed)
+    impl_.release();
+}
+
+size_t FimDevice::reserved_bytes() const { return impl_ ? impl_->reserved : 0; }
+
+template <class Real>
+bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double taper, int seed, int fs[3], int fe[3],
+                    bool structured, const double *d_points, long npoints, Real *d_field, double stats[4],
+                    hipStream_t stream, std::string &err)
+{
+    Impl &m = *impl_;
+    auto ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        err = std::string("FIM: ") + what + ": " + hipGetErrorString(e);
+        return false;
+    };
+    auto fft_ok = [&](hipfftResult r, const char *what) {
+        if (r == HIPFFT_SUCCESS) return true;
+        err = std::string("FIM: ") + what + " failed (hipfftResult " + std::to_string(int(r)) + ")";
+        return false;
+    };
+    const int n1 = s.npts[0], n2 = s.npts[1], n3 = s.npts[2], h = n1 / 2 + 1;
+    const long lines = long(n2) * n3;
+    const long total = long(n1) * lines;
+    if (total >= (long(1) << kJumpBits)) {
+        err = "FIM: internal grid too large";
+        return false;
+    }
+    if (!m.d_gen) {
+        GenTables t;
+        fim_stream::make_tables(t);
+        if (!ok(hipMalloc(&m.d_gen, sizeof(GenTables)), "hipMalloc")) return false;
+        m.reserved += sizeof(GenTables);
+        if (!ok(hipMemcpy(m.d_gen, &t, sizeof t, hipMemcpyHostToDevice), "hipMemcpy")) return false;
+        if (!ok(hipMalloc(&m.d_part, sizeof(double) * 3 * kScaleBlocks), "hipMalloc")) return false;
+        if (!ok(hipMalloc(&m.d_stat, sizeof(double) * 8), "hipMalloc")) return false;
+        m.reserved += sizeof(double) * (3 * kScaleBlocks + 8);
+        if (!ok(hipHostMalloc(reinterpret_cast<void **>(&m.h_stat), sizeof(double) * 8, hipHostMallocDefault),
+                "hipHostMalloc"))
+            return false;
+        for (auto &e : m.ev)
+            if (!ok(hipEventCreate(&e), "hipEventCreate")) return false;
+    }
+    if (!m.grow(&m.d_spec, m.spec_cap, sizeof(Cplx<Real>) * size_t(h) * size_t(lines), err)) return false;
+    if (!m.grow(&m.d_real, m.real_cap, sizeof(Real) * size_t(total), err)) return false;
+    const int npoi = static_cast<int>(s.pt.size() / 3);
+    if (npoi > 0) {
+        if (!m.grow(&m.d_pt, m.pt_cap, sizeof(Real) * s.pt.size(), err)) return false;
+        if (!ok(hipMemcpyAsync(m.d_pt, s.pt.data(), sizeof(Real) * s.pt.size(), hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync"))
+            return false;
+    }
+    // transforms: (y, z) backward c2c over every kx, then c2r along x
+    if (!m.plans || m.plan_n[0] != n1 || m.plan_n[1] != n2 || m.plan_n[2] != n3 || m.plan_real != int(sizeof(Real))) {
+        m.drop_plans();
+        if (n3 > 1) {
+            int n[2] = {n3, n2};
+            if (!fft_ok(hipfftPlanMany(&m.plan_yz, 2, n, n, h, 1, n, h, 1, FftTypes<Real>::c2c, h), "hipfftPlanMany (y, z)"))
+                return false;
+        } else {
+            int n[1] = {n2};
+            if (!fft_ok(hipfftPlanMany(&m.plan_yz, 1, n, n, h, 1, n, h, 1, FftTypes<Real>::c2c, h), "hipfftPlanMany (y)"))
+                return false;
+        }
+        int nx[1] = {n1}, in[1] = {h}, on[1] = {n1};
+        if (!fft_ok(hipfftPlanMany(&m.plan_x, 1, nx, in, 1, h, on, 1, n1, FftTypes<Real>::c2r, int(lines)),
+                    "hipfftPlanMany (x)")) {
+            (void)hipfftDestroy(m.plan_yz);
+            return false;
+        }
+        m.plans = true;
+        m.plan_n[0] = n1;
+        m.plan_n[1] = n2;
+        m.plan_n[2] = n3;
+        m.plan_real = int(sizeof(Real));
+    }
+    if (!fft_ok(hipfftSetStream(m.plan_yz, stream), "hipfftSetStream") ||
+        !fft_ok(hipfftSetStream(m.plan_x, stream), "hipfftSetStream"))
+        return false;
+
+    Cplx<Real> *S = static_cast<Cplx<Real> *>(m.d_spec);
+    Real *R = static_cast<Real *>(m.d_real);
+
+    // spectrum
+    SpecArgs<Real> a{};
+    a.n1 = n1;
+    a.n2 = n2;
+    a.n3 = n3;
+    a.h = h;
+    a.lines = lines;
+    a.lines_per_item = 8;
+    fim_stream::position0(seed, a.s1);
+    fim_stream::power(uint64_t(n1), a.line_step);
+    for (int d = 0; d < 3; ++d) {
+        a.dk[d] = s.dk[d];
+        a.cl[d] = s.cl[d];
+    }
+    a.kc = s.kc;
+    a.nu = s.nu;
+    a.cnst = s.cnst;
+    a.filter = s.filter ? 1 : 0;
+    a.acf = s.acf;
+    const long items = (lines + a.lines_per_item - 1) / a.lines_per_item;
+    const int sblocks = static_cast<int>(std::min<long>((items + 3) / 4, 2048));
+    if (!ok(hipEventRecord(m.ev[0], stream), "hipEventRecord")) return false;
+    fim_spectrum<Real><<<sblocks, 256, 0, stream>>>(S, a, m.d_gen, items);
+    if (!ok(hipGetLastError(), "fim_spectrum launch")) return false;
+    if (!ok(hipEventRecord(m.ev[1], stream), "hipEventRecord")) return false;
+    const long sym = 2 * lines;
+    fim_symmetry<Real><<<static_cast<int>((sym + 255) / 256), 256, 0, stream>>>(S, n1, n2, n3, h);
+    if (!ok(hipGetLastError(), "fim_symmetry launch")) return false;
+    if (!ok(hipEventRecord(m.ev[2], stream), "hipEventRecord")) return false;
+    if (!fft_ok(FftTypes<Real>::exec_c2c(m.plan_yz, S), "hipfftExec (y, z)")) return false;
+    fim_dc_real<Real><<<static_cast<int>((2 * lines + 255) / 256), 256, 0, stream>>>(S, n1, h, lines);
+    if (!ok(hipGetLastError(), "fim_dc_real launch")) return false;
+    if (!fft_ok(FftTypes<Real>::exec_c2r(m.plan_x, S, R), "hipfftExec (x)")) return false;
+    if (!ok(hipEventRecord(m.ev[3], stream), "hipEventRecord")) return false;
+    // scaling, statistics, rescale and taper on the internal grid
+    fim_scale<Real><<<kScaleBlocks, 256, 0, stream>>>(R, total, s.scaling, m.d_part);
+    if (!ok(hipGetLastError(), "fim_scale launch")) return false;
+    fim_finalize<Real><<<1, 256, 0, stream>>>(m.d_part, kScaleBlocks, static_cast<double>(s.sigma), m.d_stat);
+    if (!ok(hipGetLastError(), "fim_finalize launch")) return false;
+    if (rescale == 1 || npoi > 0) {
+        const int eblocks = static_cast<int>(std::min<long>((total + 255) / 256, 8192));
+        fim_epilogue<Real><<<eblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, m.d_stat, rescale == 1 ? 1 : 0,
+                                                        static_cast<const Real *>(m.d_pt), npoi,
+                                                        static_cast<Real>(mute), static_cast<Real>(taper));
+        if (!ok(hipGetLastError(), "fim_epilogue launch")) return false;
+    }
+    if (!ok(hipEventRecord(m.ev[4], stream), "hipEventRecord")) return false;
+    // nearest-neighbour values at the caller's points
+    if (structured) {
+        const int nx = fe[0] - fs[0] + 1, ny = fe[1] - fs[1] + 1, nz = fe[2] - fs[2] + 1;
+        const long nout = long(nx) * ny * nz;
+        const int mblocks = static_cast<int>(std::max<long>(1, std::min<long>((nout + 255) / 256, 8192)));
+        fim_map_grid<Real><<<mblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, d_field, nx, ny, nz, fs[0], fs[1], fs[2],
+                                                        s.ds, s.off_axis[0], s.off_axis[1], s.off_axis[2]);
+    } else {
+        const int mblocks = static_cast<int>(std::max<long>(1, std::min<long>((npoints + 255) / 256, 8192)));
+        fim_map_points<Real><<<mblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, d_field, d_points, d_points + npoints,
+                                                          d_points + 2 * npoints, npoints, s.off_axis[0],
+                                                          s.off_axis[1], s.off_axis[2]);
+    }
+    if (!ok(hipGetLastError(), "fim_map launch")) return false;
+    if (!ok(hipEventRecord(m.ev[5], stream), "hipEventRecord")) return false;
+    if (!ok(hipMemcpyAsync(m.h_stat, m.d_stat, sizeof(double) * 6, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync"))
+        return false;
+    if (!ok(hipStreamSynchronize(stream), "FIM run")) return false;
+    float ms[5] = {};
+    for (int q = 0; q < 5; ++q)
+        if (!ok(hipEventElapsedTime(&ms[q], m.ev[q], m.ev[q + 1]), "hipEventElapsedTime")) return false;
+    timings_.spectrum = ms[0] * 1e-3;
+    timings_.symmetry = ms[1] * 1e-3;
+    timings_.fft = ms[2] * 1e-3;
+    timings_.mapping = ms[4] * 1e-3;
+    timings_.total = (ms[0] + ms[1] + ms[2] + ms[3] + ms[4]) * 1e-3;
+    const double *st = m.h_stat;
+    if (rescale == 1) {
+        stats[0] = static_cast<double>(s.sigma);
+        stats[1] = 0.0;
+    } else {
+        stats[0] = st[3];
+        stats[1] = st[4];
+    }
+    stats[2] = st[1];
+    stats[3] = st[2];
+    return true;
+}
+
+template bool FimDevice::run<float>(const FimSetup<float> &, int, double, double, int, int[3], int[3], bool,
+                                    const double *, long, float *, double[4], hipStream_t, std::string &);
+template bool FimDevice::run<double>(const FimSetup<double> &, int, double, double, int, int[3], int[3], bool,
+                                     const double *, long, double *, double[4], hipStream_t, std::string &);
+
+}  // namespace scarf

@@ -8,9 +8,12 @@
 //   - points of interest are copied at initialize (the reference keeps a
 //     pointer, and its unstructured path assigns into a zero-size pointer);
 //   - errors are reported (stderr + scarf_last_error) instead of crashing:
-//     initialize twice, execute before initialize, FIM (method 0), acf = 2,
-//     von Karman with hurst <= 0;
-//   - nharm is a runtime setting (default 10000).
+//     initialize twice, execute before initialize, acf = 2, von Karman with
+//     hurst <= 0;
+//   - nharm is a runtime setting (default 10000);
+//   - FIM (method 0) runs as the reference does on one process holding the
+//     whole internal grid (fim.hpp); its rotation angles are ignored, as in
+//     the reference's default build (DIP off).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +29,7 @@
 
 #include "../../include/scarf3d.h"
 #include "../../include/scarf3d_srm.h"
+#include "fim.hpp"
 #include "srm_device.hpp"
 #include "srm_host.hpp"
 #include "srm_mfma.hpp"
@@ -51,6 +55,7 @@ struct State {
     int stream_chunk = 0;     // kept harmonics per streamed chunk, a multiple of 128 (0: growing chunks)
     std::string error;
     std::unique_ptr<scarf::Device> device;
+    std::unique_ptr<scarf::FimDevice> fim;
     std::unique_ptr<scarf::Ensemble> ensemble;
     // resident realization (scarf_srm_draw -> scarf_srm_run)
     bool table_ready = false;
@@ -236,7 +241,7 @@ int prepare(int seed)
     State &s = state();
     s.table_ready = false;
     if (!s.initialized) return fail("execute called before initialize");
-    if (s.cfg.method != 1) return fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
+    if (s.cfg.method != 1) return fail("the SRM harmonic table needs method = 1 (SRM); method 0 (FIM) runs through execute");
     s.cfg.nharm = s.nharm;
     std::string err;
     const auto t0 = std::chrono::steady_clock::now();
@@ -288,6 +293,56 @@ int run(real *d_field, real stats[], const scarf::Device::SlabHook *on_slab = nu
     return 0;
 }
 
+long field_points(const SrmConfig &cfg)
+{
+    if (!cfg.structured) return cfg.npts;
+    return long(cfg.fe[0] - cfg.fs[0] + 1) * (cfg.fe[1] - cfg.fs[1] + 1) * (cfg.fe[2] - cfg.fs[2] + 1);
+}
+
+// One FIM realization into d_field: stats(1:2) the coverage flags, (3:4)
+// standard deviation and mean of the internal field (sigma and 0 after a
+// rescale), (5:8) device seconds of the spectrum, the symmetry pass, the
+// transforms and the mapping (src/scarflib_fim.f90:614-1104, info(1:8)).
+// The field is handed to on_slab in one piece at the end.
+int execute_fim(int seed, real *d_field, real stats[], const scarf::Device::SlabHook *on_slab)
+{
+    State &s = state();
+    scarf::FimSetup<real> fs;
+    std::string err;
+    if (scarf::fim_setup<real>(s.cfg, fs, err) != 0) return fail(err);
+    if (!ensure_device()) return -1;
+    if (!s.fim) s.fim = std::make_unique<scarf::FimDevice>();
+    const double *pts = nullptr;
+    if (!s.cfg.structured) {
+        if (!s.device->upload_points<real>(s.x, s.y, s.z, s.cfg.npts, err)) return fail(err);
+        pts = s.device->points();
+    }
+    int fsi[3], fei[3];
+    for (int i = 0; i < 3; ++i) {
+        fsi[i] = s.cfg.fs[i];
+        fei[i] = s.cfg.fe[i];
+    }
+    double st[4] = {};
+    if (!s.fim->run<real>(fs, s.cfg.rescale, s.cfg.mute, s.cfg.taper, seed, fsi, fei, s.cfg.structured, pts,
+                          s.cfg.npts, d_field, st, s.device->stream(), err))
+        return fail(err);
+    if (stats) {
+        clear_stats(stats);
+        stats[0] = static_cast<real>(fs.info1);
+        stats[1] = static_cast<real>(fs.info2);
+        stats[2] = static_cast<real>(st[0]);
+        stats[3] = static_cast<real>(st[1]);
+        const scarf::FimTimings &t = s.fim->timings();
+        stats[4] = static_cast<real>(t.spectrum);
+        stats[5] = static_cast<real>(t.symmetry);
+        stats[6] = static_cast<real>(t.fft);
+        stats[7] = static_cast<real>(t.mapping);
+    }
+    if (on_slab && !(*on_slab)(0, field_points(s.cfg), err)) return fail(err);
+    s.error.clear();
+    return 0;
+}
+
 // One realization into d_field. On the MFMA kernels of a structured grid
 // the draw is streamed: the device starts on the first chunk of harmonics
 // while the host draws the next (the reference draws each harmonic before
@@ -298,7 +353,7 @@ int execute_realization(int seed, real *d_field, real stats[], const scarf::Devi
     State &s = state();
     s.table_ready = false;
     if (!s.initialized) return fail("execute called before initialize");
-    if (s.cfg.method != 1) return fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
+    if (s.cfg.method == 0) return execute_fim(seed, d_field, stats, on_slab);
     s.cfg.nharm = s.nharm;
     const int kernel = resolve_kernel(s.cfg);
     const bool mfma = kernel == scarf::kKernelMfmaBf16x3 || kernel == scarf::kKernelMfmaF16x2 ||
@@ -347,12 +402,6 @@ int execute_realization(int seed, real *d_field, real stats[], const scarf::Devi
     report_stats(stats, st);
     s.error.clear();
     return 0;
-}
-
-long field_points(const SrmConfig &cfg)
-{
-    if (!cfg.structured) return cfg.npts;
-    return long(cfg.fe[0] - cfg.fs[0] + 1) * (cfg.fe[1] - cfg.fs[1] + 1) * (cfg.fe[2] - cfg.fs[2] + 1);
 }
 
 // Runs f; an exception from it is reported as the call's error instead of
@@ -554,14 +603,17 @@ static void scarf_execute_body(const int seed, real *field, real stats[])
         fail("execute called before initialize");
         return;
     }
-    if (s.cfg.method != 1) {
-        fail("method 0 (FIM) is not provided by this library; set method = 1 (SRM)");
-        return;
-    }
     {  // parameter errors are reported before any device work
         s.cfg.nharm = s.nharm;
         std::string e;
-        if ((kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, e) : scarf::srm_setup<float>(s.cfg, s.setup, e)) != 0) {
+        int rc;
+        if (s.cfg.method == 0) {
+            scarf::FimSetup<real> fs;
+            rc = scarf::fim_setup<real>(s.cfg, fs, e);
+        } else {
+            rc = kDouble ? scarf::srm_setup<double>(s.cfg, s.setup, e) : scarf::srm_setup<float>(s.cfg, s.setup, e);
+        }
+        if (rc != 0) {
             fail(e);
             return;
         }

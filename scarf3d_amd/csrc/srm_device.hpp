@@ -126,6 +126,11 @@ public:
 
     int nharm_resident() const { return nharm_; }
 
+    // FP64 coordinates of the last uploaded point list (x, y, z blocks of
+    // points_count() values each)
+    const double *points() const { return d_pts_; }
+    long points_count() const { return npts_; }
+
     // tile autotuner of the MFMA kernels (shared with ensembles)
     TileTuner &tuner();
     // x-tile width (32 * nbx) and column tile tk of the last MFMA run

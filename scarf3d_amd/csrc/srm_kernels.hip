@@ -354,23 +354,8 @@ __global__ __launch_bounds__(256) void direct_points(const double *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // Epilogue: rescale (field*s - mean, src/scarflib_srm.f90:897-912) and the
-// mute/taper of every point of interest in order
-// (src/scarflib_common.f90:881-994), at `real` precision without
-// contraction, as the reference computes them on the host.
-template <class Real>
-__device__ inline Real taper_factor(Real val, Real dx2, Real dy2, Real dz2, Real mute, Real taper)
-{
-#pragma clang fp contract(off)
-    const Real d = sqrt(dx2 + dy2 + dz2);
-    const Real pi = static_cast<Real>(3.141592653589793);
-    if (d <= mute) return Real(0);
-    if (d > mute && d <= mute + taper) {
-        const Real t = (d - mute) / taper;
-        return val * (Real(0.5) - Real(0.5) * cos(t * pi));
-    }
-    return val;
-}
-
+// mute/taper of every point of interest in order (taper_factor), at `real`
+// precision without contraction, as the reference computes them on the host.
 template <class Real>
 __global__ __launch_bounds__(256) void epilogue_grid(Real *__restrict__ field, int nx, int ny, int nz, int ox,
                                                      int oy, int oz, double ds_, const double *__restrict__ stat,

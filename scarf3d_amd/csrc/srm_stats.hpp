@@ -87,4 +87,22 @@ __device__ inline Moments local_moments(const double (&v)[N], const bool (&ok)[N
     return m;
 }
 
+// Mute/taper of one value at squared axis distances (dx2, dy2, dz2) from a
+// point of interest (src/scarflib_common.f90:881-994): zero within `mute`, a
+// raised-cosine ramp out to mute + taper, unchanged beyond. `real` precision
+// without contraction, as the reference computes it on the host.
+template <class Real>
+__device__ inline Real taper_factor(Real val, Real dx2, Real dy2, Real dz2, Real mute, Real taper)
+{
+#pragma clang fp contract(off)
+    const Real d = sqrt(dx2 + dy2 + dz2);
+    const Real pi = static_cast<Real>(3.141592653589793);
+    if (d <= mute) return Real(0);
+    if (d > mute && d <= mute + taper) {
+        const Real t = (d - mute) / taper;
+        return val * (Real(0.5) - Real(0.5) * cos(t * pi));
+    }
+    return val;
+}
+
 }  // namespace scarf

@@ -74,6 +74,13 @@ public:
         r1_ = static_cast<int64_t>((p[2] * r0 % kM + p[3] * r1 % kM) % kM);
     }
 
+    // the state pair (r0 = latest value of the recursion, r1 = the one before)
+    void state(uint32_t &r0, uint32_t &r1) const
+    {
+        r0 = static_cast<uint32_t>(r0_);
+        r1 = static_cast<uint32_t>(r1_);
+    }
+
 private:
     static constexpr uint64_t kM = static_cast<uint64_t>(kModulus);
 

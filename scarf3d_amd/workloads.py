@@ -58,10 +58,12 @@ def init_library(lib, cfg: dict) -> None:
     """Initialize `lib` (scarf3d_amd.Library) so that its object holds
     exactly `cfg`: options are passed only where the presence rules of
     src/c_binding.c:89-103 forward them (nc/fc of a structured grid only when
-    they differ from the block's own extents)."""
+    they differ from the block's own extents). cfg["method"] selects the
+    algorithm (default 1, SRM; 0 is FIM, with cfg["pad"])."""
     nd = cfg["nd"]
-    opts = dict(method=1, hurst=cfg.get("hurst", 0.0), alpha=cfg.get("alpha", 0.0), beta=cfg.get("beta", 0.0),
-                gamma=cfg.get("gamma", 0.0), rescale=cfg.get("rescale", 0))
+    opts = dict(method=cfg.get("method", 1), hurst=cfg.get("hurst", 0.0), alpha=cfg.get("alpha", 0.0),
+                beta=cfg.get("beta", 0.0), gamma=cfg.get("gamma", 0.0), rescale=cfg.get("rescale", 0),
+                pad=cfg.get("pad", 0))
     if cfg.get("mute", -999.0) > 0:
         opts["mute"] = cfg["mute"]
     if cfg.get("taper", -999.0) > 0:

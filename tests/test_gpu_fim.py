@@ -1,0 +1,123 @@
+"""GPU parity of the FIM path (method 0) against the FP64 FIM oracle
+(oracle/fim_oracle.py) on the same seed.
+
+Bar: max|field - oracle| <= 1e-5 of the field's standard deviation for the
+float library (FP32 spectrum and transforms), 1e-10 for the double library;
+info(1:2) exact, info(3) relative 1e-5 (1e-10), info(4) absolute 1e-6
+(1e-12) of the standard deviation (the mean of the internal field is zero up
+to rounding: its kx = ky = kz = 0 coefficient is set to zero).
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from scarf3d_amd.workloads import init_library, structured_cfg, unstructured_cfg
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import fim_oracle as fo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f": (1e-5, 1e-5, 1e-6), "d": (1e-10, 1e-10, 1e-12)}
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _fim(cfg, pad=0):
+    c = dict(cfg)
+    c["method"] = 0
+    c["pad"] = pad
+    return c
+
+
+def _run(lib, cfg, seed):
+    init_library(lib, cfg)
+    field, stats = lib.execute(seed)
+    err = lib.last_error()
+    assert err == "", err
+    return field, stats
+
+
+def _check(lib, cfg, seed, flav):
+    real = np.float32 if flav == "f" else np.float64
+    ref = fo.realization(cfg, seed, real)
+    field, stats = _run(lib, cfg, seed)
+    want = ref["field"]
+    got = field.reshape(-1, order="F").astype(np.float64)
+    sd = ref["info"][2]
+    err = np.abs(got - want).max() / sd
+    t_field, t_std, t_mean = TOL[flav]
+    print(f"  internal grid {ref['setup']['npts']}, max|err|/sd = {err:.3e}, info3 {stats[2]:.7g} vs {sd:.7g}, "
+          f"info4 {stats[3]:.3e} vs {ref['info'][3]:.3e}", flush=True)
+    assert err <= t_field
+    assert stats[0] == ref["info"][0] and stats[1] == ref["info"][1]
+    assert abs(float(stats[2]) / sd - 1.0) <= t_std
+    assert abs(float(stats[3]) - ref["info"][3]) <= t_mean * sd
+    assert np.all(stats[4:8] >= 0)
+    return field, stats, ref
+
+
+GRID_CASES = [
+    ("vk_3d", dict(n=(40, 36, 20), ds=50.0, cl=(500.0, 400.0, 300.0), hurst=0.25, sigma=0.05), 0),
+    ("gauss_pad", dict(n=(33, 30, 17), ds=50.0, acf=1, cl=(300.0, 300.0, 100.0), sigma=0.1), 1),
+    ("vk_2d", dict(n=(64, 48, 1), ds=25.0, cl=(400.0, 300.0, 0.0), hurst=0.5, sigma=0.05), 0),
+    ("filter_ds_ne_dh", dict(n=(30, 26, 22), ds=20.0, dh=10.0, cl=(200.0, 200.0, 200.0), hurst=0.3), 0),
+    ("bluestein_sizes", dict(n=(105, 13, 7), ds=50.0, cl=(400.0, 100.0, 100.0), hurst=0.2), 0),
+    ("rescale_taper", dict(n=(32, 32, 24), ds=50.0, cl=(500.0, 500.0, 500.0), hurst=0.25, rescale=1,
+                           poi=np.array([[400.0, 700.0], [800.0, 300.0], [600.0, 500.0]]), mute=100.0,
+                           taper=400.0), 1),
+]
+
+
+@pytest.mark.parametrize("name,kw,pad", GRID_CASES, ids=[c[0] for c in GRID_CASES])
+def test_fim_grid_matches_oracle(lib_f, name, kw, pad):
+    cfg = _fim(structured_cfg(real=np.float32, **kw), pad)
+    _say(name, "libscarf3df", kw, "pad", pad)
+    _check(lib_f, cfg, 1235, "f")
+
+
+@pytest.mark.parametrize("name,kw,pad", [GRID_CASES[0], GRID_CASES[2], GRID_CASES[5]],
+                         ids=[GRID_CASES[i][0] for i in (0, 2, 5)])
+def test_fim_grid_double_library(lib_d, name, kw, pad):
+    cfg = _fim(structured_cfg(real=np.float64, **kw), pad)
+    _say(name, "libscarf3d", kw, "pad", pad)
+    _check(lib_d, cfg, 77, "d")
+
+
+def test_fim_block_of_larger_model(lib_f):
+    """A block fs..fe of a larger model (nc/fc of the whole model, as each MPI
+    rank of the reference passes them): the internal grid covers the model,
+    the block takes its part."""
+    cfg = structured_cfg(n=(20, 16, 12), fs=(11, 5, 3), ds=40.0, cl=(300.0, 300.0, 300.0), hurst=0.3,
+                         real=np.float32)
+    cfg["nc"] = (0.0, 0.0, 0.0)
+    cfg["fc"] = (59 * 40.0, 39 * 40.0, 29 * 40.0)
+    cfg = _fim(cfg)
+    _say("block (11..30, 5..20, 3..14) of 60 x 40 x 30")
+    _check(lib_f, cfg, 4, "f")
+
+
+@pytest.mark.parametrize("nd", [3, 2])
+def test_fim_points_match_oracle(lib_f, nd):
+    cfg = _fim(unstructured_cfg(npts=3000, box=(4000.0, 3000.0, 2000.0), dh=50.0, acf=0, hurst=0.25,
+                                cl=(600.0, 500.0, 400.0), sigma=0.08, seed=21, real=np.float32, nd=nd), 1)
+    _say("points", nd, "D")
+    _check(lib_f, cfg, 8, "f")
+
+
+def test_fim_repeatable_and_seeded(lib_f):
+    cfg = _fim(structured_cfg(n=(24, 20, 16), ds=50.0, cl=(300.0, 300.0, 300.0), hurst=0.2, real=np.float32))
+    _say("repeat / seeds")
+    a, _ = _run(lib_f, cfg, 5)
+    lib_f.finalize()
+    b, _ = _run(lib_f, cfg, 5)
+    lib_f.finalize()
+    c, _ = _run(lib_f, cfg, 6)
+    np.testing.assert_array_equal(a, b)
+    assert np.abs(a - c).max() > 0

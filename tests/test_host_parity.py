@@ -250,8 +250,8 @@ def test_initialize_twice_and_execute_before_initialize(lib_f):
 
 
 @pytest.mark.parametrize("opts,msg", [
-    (dict(), "FIM"),                              # method 0 is FIM
-    (dict(method=1), "hurst"),                    # von Karman needs hurst > 0
+    (dict(), "hurst"),                            # FIM (method 0, the default), von Karman needs hurst > 0
+    (dict(method=1), "hurst"),                    # SRM likewise
 ])
 def test_parameter_errors_reported(lib_f, opts, msg):
     lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 0, [2000.0] * 3, 0.05, lib_f.opt(**opts))
@@ -260,8 +260,10 @@ def test_parameter_errors_reported(lib_f, opts, msg):
     assert np.all(field == 0) and np.all(stats == 0)
 
 
-def test_user_defined_psd_rejected(lib_f):
-    lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 2, [2000.0] * 3, 0.05, lib_f.opt(method=1, hurst=0.3))
+@pytest.mark.parametrize("method", [0, 1])
+def test_user_defined_psd_rejected(lib_f, method):
+    lib_f.cart_initialize(3, [1, 1, 1], [16, 16, 16], 50.0, 2, [2000.0] * 3, 0.05,
+                          lib_f.opt(method=method, hurst=0.3))
     lib_f.execute(1)
     assert "acf" in lib_f.last_error()
 
