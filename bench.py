@@ -220,15 +220,16 @@ FIM_CASE = dict(n=(500, 500, 500), ds=50.0, acf=0, hurst=0.25, cl=(2000.0,) * 3,
 
 
 def fim_bytes(npts, nout):
-    """Algorithmic HBM bytes of one FIM realization (float library): the half
-    spectrum written once (8 B per complex), read and written once by the
-    (y, z) transforms (16 B) and read by the x transform (8 B), the internal
-    field written by it (4 B), read and written by the scaling (8 B), and the
-    external field written by the mapping with one internal value read per
-    output point (8 B)."""
+    """Least HBM bytes of one FIM realization (float library): the half
+    spectrum written once (8 B per complex value) and read once by the
+    transform, which writes the internal field once (4 B per point); the
+    statistics read it once (4 B); the mapping reads one internal value and
+    writes one output value per output point (8 B). The transform itself
+    takes several passes (a 3D FFT of non-power-of-two lengths), so the
+    fraction of HBM bandwidth this gives is a floor."""
     n1, n2, n3 = npts
     half = (n1 // 2 + 1) * n2 * n3
-    return 8 * half + 16 * half + 8 * half + 4 * n1 * n2 * n3 + 8 * n1 * n2 * n3 + 8 * nout
+    return 8 * half + 8 * half + 4 * n1 * n2 * n3 + 4 * n1 * n2 * n3 + 8 * nout
 
 
 def measure_fim(steps, warmup, cpu=True):
@@ -285,8 +286,9 @@ def measure_fim(steps, warmup, cpu=True):
            "internal_mpts_per_s": _sig(nint_ / (dev * 1e-3) / 1e6),
            "hbm": {"algorithmic_bytes": B, "achieved_gbs": _sig(B / (dev * 1e-3) / 1e9), "peak_gbs": PEAK_HBM_GBS,
                    "frac": _sig(B / (dev * 1e-3) / 1e9 / PEAK_HBM_GBS)},
-           "note": "scarf_execute_device wall clock per call (spectrum, symmetry, hipFFT (y, z) c2c + x c2r, "
-                   "statistics, mapping); stage ms from the library's stats(5:8), HIP events"}
+           "note": "scarf_execute_device wall clock per call (spectrum, symmetry, one hipFFT 3D c2r, statistics, "
+                   "mapping); stage ms from the library's stats(5:8), HIP events; hbm.frac counts the least "
+                   "bytes of each stage (fim_bytes)"}
     log(f"[bench] FIM {c['n']} (internal {npts}): first call {first_ms:.1f} ms, {dev:.2f} ms per call "
         f"(spectrum {part[0]:.2f}, symmetry {part[1]:.3f}, transforms {part[2]:.2f}, mapping {part[3]:.2f} ms), "
         f"host-array call {float(np.mean(hms)):.1f} ms")

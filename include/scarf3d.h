@@ -1,5 +1,5 @@
 /*
- * scarf3d.h - public C/C++ interface of the MI355X SCARF3D SRM library.
+ * scarf3d.h - public C/C++ interface of the MI355X SCARF3D library (SRM and FIM).
  *
  * Drop-in for the reference's src/scarf3d.h:36-162: the same `real` rule,
  * the same struct scarf_opt layout and the same entry points, exported with C
@@ -19,8 +19,10 @@
  * Optional members of scarf_opt count as given when: method == 1, hurst > 0,
  * dh > 0 (structured) / ds > 0 (unstructured), poi != NULL with npoi > 0,
  * mute > 0, taper > 0, rescale == 1, pad == 1, nc and fc both non-NULL,
- * alpha/beta/gamma != 0 (src/c_binding.c:89-103, :116-130). method 0 selects
- * FIM, which this library does not provide: execute then reports an error.
+ * alpha/beta/gamma != 0 (src/c_binding.c:89-103, :116-130). method 0 (the
+ * default) selects FIM, method 1 SRM. FIM runs as the reference's default
+ * build (DIP off) runs it on one process: the rotation angles are ignored
+ * (src/scarflib_fim.f90:614-1104, :141-612).
  */
 #ifndef SCARF3D_H_
 #define SCARF3D_H_

@@ -9,9 +9,9 @@
 //   interest; the generator tables (fim_stream.hpp: 18 KB of powers of the
 //   primitive root, the companion-matrix powers), uploaded once per process.
 // Work per internal point (bytes of HBM): spectrum 8 (written), symmetry
-// reads/writes 2 planes, transforms ~3 passes over S plus R, statistics 4+4
-// (R read, scaled R written), epilogue 8 if active, mapping 4 per output
-// point plus the gathered R values.
+// reads/writes 2 planes, the c2r transform ~3 passes over S plus R, statistics 4
+// (R read), epilogue 8 if active, mapping 4 per output point plus the
+// gathered R values.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
@@ -106,8 +106,17 @@ __global__ __launch_bounds__(256) void fim_spectrum(Cplx<Real> *__restrict__ S, 
 //                           region) is replaced by the exchange of the j = 0
 //                           column, whose subarray types overwrite it
 //                           (:2866-2960), so the line is conjugated in place.
-// Every position is written by at most one rule and every value read comes
-// from a position no rule writes, so one pass in place is exact.
+// After these rules each plane is Hermitian (S(-j,-k) = conj S(j,k)) except
+// that 3D line, and the reference's c2r along x reads only the real parts of
+// the planes' values after the (y, z) transforms. The real part of a 2D
+// transform is the transform of the plane's Hermitian part
+// (S(j,k) + conj S(-j,-k)) / 2, so the line's pairs (j, n2 - j), 0 < j < jn,
+// are replaced by v = (B(j,0) + B(n2-j,0)) / 2 and conj(v): the planes are
+// then exactly Hermitian, one 3D c2r transform reproduces the reference's
+// z, y, x sequence, and no transform depends on how it treats imaginary parts
+// that should be zero.
+// Every position is written by one thread at most, and each thread reads only
+// positions that no other thread writes, so one pass in place is exact.
 template <class Real>
 __global__ __launch_bounds__(256) void fim_symmetry(Cplx<Real> *__restrict__ S, int n1, int n2, int n3, int h)
 {
@@ -129,55 +138,63 @@ __global__ __launch_bounds__(256) void fim_symmetry(Cplx<Real> *__restrict__ S, 
         if (j == jn && k == 0) v.im = Real(0);
         return;
     }
-    const int jp = j == 0 ? 0 : n2 - j;
     if (k > kn) {
-        const Cplx<Real> b = at(jp, n3 - k);
+        const Cplx<Real> b = at(j == 0 ? 0 : n2 - j, n3 - k);
         v = Cplx<Real>{b.re, -b.im};
     } else if (j > jn && k == kn) {
         const Cplx<Real> b = at(n2 - j, kn);
         v = Cplx<Real>{b.re, -b.im};
-    } else if (j > jn && k == 0) {
-        const Cplx<Real> b = n3 == 1 ? at(n2 - j, 0) : v;
+    } else if (k == 0 && n3 == 1 && j > jn) {
+        const Cplx<Real> b = at(n2 - j, 0);
         v = Cplx<Real>{b.re, -b.im};
+    } else if (k == 0 && n3 > 1 && j > 0 && j < jn) {
+        // the pair (j, n2 - j) of the in-place line, by its Hermitian part
+        Cplx<Real> &w = at(n2 - j, 0);
+        const Cplx<Real> hp{(v.re + w.re) / Real(2), (v.im + w.im) / Real(2)};
+        v = hp;
+        w = Cplx<Real>{hp.re, -hp.im};
     }
 }
 
-// After the (y, z) transforms: the c2r transform along x reads only the real
-// parts of the kx = 0 and Nyquist coefficients of every x-line (the
-// half-complex form of FFTW's c2r has no slot for their imaginary parts), so
-// they are set to zero explicitly and any c2r implementation agrees.
+// Moments of buffer = REAL(spec) * scaling (src/scarflib_fim.f90:955-961):
+// each value is the `real` product the reference stores, accumulated in
+// FP64 as a sum and a sum of squares per thread (the internal field has zero
+// mean up to rounding: its kx = ky = kz = 0 coefficient is zero, so the sum
+// of squares carries the variance without cancellation), then merged per
+// block as (count, mean, M2). R itself is left unscaled: the pass that next
+// writes or reads each value (epilogue or mapping) applies the same product.
 template <class Real>
-__global__ __launch_bounds__(256) void fim_dc_real(Cplx<Real> *__restrict__ S, int n1, int h, long lines)
+__global__ __launch_bounds__(256) void fim_moments(const Real *__restrict__ R, long n, Real scaling,
+                                                   double *__restrict__ part)
 {
-    const long e = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (e >= 2 * lines) return;
-    const long line = e >> 1;
-    const int i = (e & 1) ? n1 / 2 : 0;
-    S[size_t(i) + size_t(h) * size_t(line)].im = Real(0);
-}
-
-// buffer = REAL(spec) * scaling (src/scarflib_fim.f90:955-961) and the
-// block moments of the scaled values, FP64 (count, mean, M2 per block)
-template <class Real>
-__global__ __launch_bounds__(256) void fim_scale(Real *__restrict__ R, long n, Real scaling, double *__restrict__ part)
-{
-    Moments m{0.0, 0.0, 0.0};
+    double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
     const long stride = long(gridDim.x) * blockDim.x;
-    for (long base = blockIdx.x * long(blockDim.x) + threadIdx.x; base < n; base += 8 * stride) {
-        double v[8];
-        bool ok[8];
+    long e = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    for (; e + 3 * stride < n; e += 4 * stride) {  // four loads in flight per thread
+        Real x[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long e = base + q * stride;
-            ok[q] = e < n;
-            v[q] = 0.0;
-            if (ok[q]) {
-                const Real x = R[e] * scaling;
-                R[e] = x;
-                v[q] = static_cast<double>(x);
-            }
+        for (int q = 0; q < 4; ++q) x[q] = R[e + q * stride];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double v = static_cast<double>(x[q] * scaling);
+            s1[q] += v;
+            s2[q] += v * v;
         }
-        merge_moments(m, local_moments(v, ok));
+    }
+    for (; e < n; e += stride) {
+        const double v = static_cast<double>(R[e] * scaling);
+        s1[0] += v;
+        s2[0] += v * v;
+    }
+    const long mine = n > blockIdx.x * long(blockDim.x) + threadIdx.x
+                          ? (n - 1 - (blockIdx.x * long(blockDim.x) + threadIdx.x)) / stride + 1
+                          : 0;
+    const double c = static_cast<double>(mine);
+    const double t1 = (s1[0] + s1[1]) + (s1[2] + s1[3]), t2 = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+    Moments m{c, 0.0, 0.0};
+    if (c > 0.0) {
+        m.mean = t1 / c;
+        m.m2 = t2 - t1 * m.mean;
     }
     block_moments(m, part + 3 * blockIdx.x);
 }
@@ -220,72 +237,104 @@ __global__ __launch_bounds__(256) void fim_finalize(const double *__restrict__ p
     }
 }
 
-// rescale (buffer * (sigma / info(3)) - info(4)) and the mute/taper of every
-// point of interest on the internal grid, positions (i - 1) dh
-// (src/scarflib_fim.f90:1014-1059, tapering_structured with ls = 1)
+// scaling, rescale (buffer * (sigma / info(3)) - info(4)) and the mute/taper
+// of every point of interest on the internal grid, positions (i - 1) dh
+// (src/scarflib_fim.f90:955-961, :1014-1059, tapering_structured with
+// ls = 1). Blocks walk the x-lines with a grid stride.
 template <class Real>
-__global__ __launch_bounds__(256) void fim_epilogue(Real *__restrict__ R, int n1, int n2, int n3, Real dh,
-                                                    const double *__restrict__ stat, int rescale,
+__global__ __launch_bounds__(256) void fim_epilogue(Real *__restrict__ R, int n1, int n2, int lines, Real scaling,
+                                                    Real dh, const double *__restrict__ stat, int rescale,
                                                     const Real *__restrict__ pt, int npoi, Real mute, Real taper)
 {
 #pragma clang fp contract(off)
-    const long n = long(n1) * n2 * n3;
     const Real sc = static_cast<Real>(stat[5]), mu = static_cast<Real>(stat[4]);
-    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
-        Real v = R[e];
-        if (rescale) v = v * sc - mu;
-        if (npoi > 0) {
-            const int i = int(e % n1), j = int((e / n1) % n2), k = int(e / (long(n1) * n2));
+    for (int line = blockIdx.x; line < lines; line += gridDim.x) {
+        const int j = line % n2, k = line / n2;
+        Real *row = R + size_t(line) * n1;
+        for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+            Real v = row[i] * scaling;
+            if (rescale) v = v * sc - mu;
             for (int q = 0; q < npoi; ++q) {
                 const Real dx = static_cast<Real>(i) * dh - pt[3 * q];
                 const Real dy = static_cast<Real>(j) * dh - pt[3 * q + 1];
                 const Real dz = static_cast<Real>(k) * dh - pt[3 * q + 2];
                 v = taper_factor(v, dx * dx, dy * dy, dz * dz, mute, taper);
             }
+            row[i] = v;
         }
-        R[e] = v;
     }
 }
 
-// Nearest internal value of a point at internal coordinates (x, y, z)
-// relative to off_axis (order_points_* and interpolate,
-// src/scarflib_fim.f90:1514-1704, :3086-3120, one process, identity
-// rotation): c = (x / dh + 3) - 1 along each axis, inside when
-// 1 <= c < npts + 1, value delta(NINT(c) - 1) of the halo-extended field
-// (index 0 is the halo: zero along x and y, the last z-plane along z, which
-// is periodic in the reference's process topology, :110, :3013-3080);
-// outside points stay 0.
+// 1-based internal index along one axis for coordinate x, 0 for the halo,
+// -1 outside
 template <class Real>
-__device__ inline Real nearest(const Real *__restrict__ R, int n1, int n2, int n3, Real dh, Real x, Real y, Real z)
+__device__ inline int axis_index(Real x, Real dh, int np)
 {
 #pragma clang fp contract(off)
-    const Real c[3] = {(x / dh + Real(3)) - Real(1), (y / dh + Real(3)) - Real(1), (z / dh + Real(3)) - Real(1)};
-    const int np[3] = {n1, n2, n3};
-    long idx[3];
-    for (int d = 0; d < 3; ++d) {
-        if (!(c[d] >= Real(1) && c[d] < static_cast<Real>(np[d] + 1))) return Real(0);
-        idx[d] = lround(c[d]) - 1;  // 0 (halo) .. npts
-    }
-    if (idx[0] == 0 || idx[1] == 0) return Real(0);
-    if (idx[2] == 0) idx[2] = n3;
-    return R[size_t(idx[0] - 1) + size_t(n1) * (size_t(idx[1] - 1) + size_t(n2) * size_t(idx[2] - 1))];
+    const Real c = (x / dh + Real(3)) - Real(1);
+    if (!(c >= Real(1) && c < static_cast<Real>(np + 1))) return -1;
+    return static_cast<int>(lround(c)) - 1;
+}
+// value at 1-based axis indices (x, y halo 0: zero; z halo 0: the last plane)
+// times `scale` (the buffer's scaling when the internal field is unscaled)
+template <class Real>
+__device__ inline Real gather(const Real *__restrict__ R, int n1, int n2, int n3, int ix, int iy, int iz, Real scale)
+{
+    if (ix <= 0 || iy <= 0 || iz < 0) return Real(0);
+    if (iz == 0) iz = n3;
+    return R[size_t(ix - 1) + size_t(n1) * (size_t(iy - 1) + size_t(n2) * size_t(iz - 1))] * scale;
 }
 
-// structured block fs..fe of step ds: external point (fs - 1 + i0) ds
-// (grid_mapping_struct / order_points_struct, src/scarflib_fim.f90:1303-1512, :1598-1700)
+// Per-axis internal indices of a structured block fs..fe of step ds:
+// external point (fs - 1 + i0) ds, axis_index of it (grid_mapping_struct /
+// order_points_struct, src/scarflib_fim.f90:1303-1512, :1598-1700). The
+// nearest node along each axis depends on that axis' coordinate only, so
+// the block's nx + ny + nz indices are computed once; tab holds x, then y,
+// then z.
 template <class Real>
-__global__ __launch_bounds__(256) void fim_map_grid(const Real *__restrict__ R, int n1, int n2, int n3, Real dh,
-                                                    Real *__restrict__ out, int nx, int ny, int nz, int fx, int fy,
-                                                    int fz, Real ds, Real ox, Real oy, Real oz)
+__global__ __launch_bounds__(256) void fim_axis_tables(int *__restrict__ tab, int nx, int ny, int nz, int fx, int fy,
+                                                       int fz, Real ds, Real dh, Real ox, Real oy, Real oz, int n1,
+                                                       int n2, int n3)
 {
 #pragma clang fp contract(off)
-    const long n = long(nx) * ny * nz;
-    for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
-        const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (long(nx) * ny));
-        const Real x = static_cast<Real>(fx - 1 + i) * ds - ox;
-        const Real y = static_cast<Real>(fy - 1 + j) * ds - oy;
-        const Real z = static_cast<Real>(fz - 1 + k) * ds - oz;
-        out[e] = nearest(R, n1, n2, n3, dh, x, y, z);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nx) {
+        tab[e] = axis_index(static_cast<Real>(fx - 1 + e) * ds - ox, dh, n1);
+    } else if (e < nx + ny) {
+        const int j = e - nx;
+        tab[e] = axis_index(static_cast<Real>(fy - 1 + j) * ds - oy, dh, n2);
+    } else if (e < nx + ny + nz) {
+        const int k = e - nx - ny;
+        tab[e] = axis_index(static_cast<Real>(fz - 1 + k) * ds - oz, dh, n3);
+    }
+}
+
+// The block's values: each block walks groups of 4 output x-lines
+// (row = j + ny k) with a grid stride, its threads along x; the four rows'
+// gathers are independent loads in flight together
+template <class Real>
+__global__ __launch_bounds__(256) void fim_map_grid(const Real *__restrict__ R, int n1, int n2, int n3, Real scale,
+                                                    Real *__restrict__ out, const int *__restrict__ tab, int nx,
+                                                    int ny, int rows)
+{
+    constexpr int G = 4;
+    for (int l0 = blockIdx.x * G; l0 < rows; l0 += gridDim.x * G) {
+        int iy[G], iz[G];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int line = min(l0 + r, rows - 1);
+            iy[r] = tab[nx + line % ny];
+            iz[r] = tab[nx + ny + line / ny];
+        }
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const int ix = tab[i];
+            Real v[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) v[r] = gather(R, n1, n2, n3, ix, iy[r], iz[r], scale);
+#pragma unroll
+            for (int r = 0; r < G; ++r)
+                if (l0 + r < rows) out[size_t(l0 + r) * nx + i] = v[r];
+        }
     }
 }
 
@@ -293,16 +342,17 @@ __global__ __launch_bounds__(256) void fim_map_grid(const Real *__restrict__ R, 
 // (grid_mapping_unstruct / order_points_unstruct, src/scarflib_fim.f90:1110-1300, :1514-1596)
 template <class Real>
 __global__ __launch_bounds__(256) void fim_map_points(const Real *__restrict__ R, int n1, int n2, int n3, Real dh,
-                                                      Real *__restrict__ out, const double *__restrict__ px,
-                                                      const double *__restrict__ py, const double *__restrict__ pz,
-                                                      long n, Real ox, Real oy, Real oz)
+                                                      Real scale, Real *__restrict__ out,
+                                                      const double *__restrict__ px, const double *__restrict__ py,
+                                                      const double *__restrict__ pz, long n, Real ox, Real oy,
+                                                      Real oz)
 {
 #pragma clang fp contract(off)
     for (long e = blockIdx.x * long(blockDim.x) + threadIdx.x; e < n; e += long(gridDim.x) * blockDim.x) {
-        const Real x = static_cast<Real>(px[e]) - ox;
-        const Real y = static_cast<Real>(py[e]) - oy;
-        const Real z = static_cast<Real>(pz[e]) - oz;
-        out[e] = nearest(R, n1, n2, n3, dh, x, y, z);
+        const int ix = axis_index(static_cast<Real>(px[e]) - ox, dh, n1);
+        const int iy = axis_index(static_cast<Real>(py[e]) - oy, dh, n2);
+        const int iz = axis_index(static_cast<Real>(pz[e]) - oz, dh, n3);
+        out[e] = (ix < 0 || iy < 0) ? Real(0) : gather(R, n1, n2, n3, ix, iy, iz, scale);
     }
 }
 
@@ -310,18 +360,17 @@ template <class Real>
 struct FftTypes;
 template <>
 struct FftTypes<float> {
-    static constexpr hipfftType c2c = HIPFFT_C2C, c2r = HIPFFT_C2R;
-    static hipfftResult exec_c2c(hipfftHandle p, void *d) { return hipfftExecC2C(p, (hipfftComplex *)d, (hipfftComplex *)d, HIPFFT_BACKWARD); }
+    static constexpr hipfftType c2r = HIPFFT_C2R;
     static hipfftResult exec_c2r(hipfftHandle p, void *in, void *out) { return hipfftExecC2R(p, (hipfftComplex *)in, (hipfftReal *)out); }
 };
 template <>
 struct FftTypes<double> {
-    static constexpr hipfftType c2c = HIPFFT_Z2Z, c2r = HIPFFT_Z2D;
-    static hipfftResult exec_c2c(hipfftHandle p, void *d) { return hipfftExecZ2Z(p, (hipfftDoubleComplex *)d, (hipfftDoubleComplex *)d, HIPFFT_BACKWARD); }
+    static constexpr hipfftType c2r = HIPFFT_Z2D;
     static hipfftResult exec_c2r(hipfftHandle p, void *in, void *out) { return hipfftExecZ2D(p, (hipfftDoubleComplex *)in, (hipfftDoubleReal *)out); }
 };
 
-constexpr int kScaleBlocks = 1024;
+constexpr int kScaleBlocks = 2048;  // 8 blocks of 256 per compute unit
+constexpr long kRowBlocks = 8192;   // row-walking passes: 32 blocks per compute unit
 
 }  // namespace
 
@@ -334,7 +383,9 @@ struct FimDevice::Impl {
     double *d_part = nullptr, *d_stat = nullptr, *h_stat = nullptr;
     void *d_pt = nullptr;
     size_t pt_cap = 0;
-    hipfftHandle plan_yz = 0, plan_x = 0;
+    int *d_tab = nullptr;  // per-axis nearest indices of a structured block
+    size_t tab_cap = 0;
+    hipfftHandle plan = 0;  // c2r over the whole internal grid
     bool plans = false;
     int plan_n[3] = {0, 0, 0};
     int plan_real = 0;  // sizeof(Real) of the plans
@@ -363,8 +414,7 @@ struct FimDevice::Impl {
     void drop_plans()
     {
         if (!plans) return;
-        (void)hipfftDestroy(plan_yz);
-        (void)hipfftDestroy(plan_x);
+        (void)hipfftDestroy(plan);
         plans = false;
     }
 };
@@ -399,7 +449,7 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
     const int n1 = s.npts[0], n2 = s.npts[1], n3 = s.npts[2], h = n1 / 2 + 1;
     const long lines = long(n2) * n3;
     const long total = long(n1) * lines;
-    if (total >= (long(1) << kJumpBits)) {
+    if (total >= (long(1) << kJumpBits) || lines >= (long(1) << 31)) {
         err = "FIM: internal grid too large";
         return false;
     }
@@ -427,33 +477,20 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
                 "hipMemcpyAsync"))
             return false;
     }
-    // transforms: (y, z) backward c2c over every kx, then c2r along x
+    // transform: one c2r over the internal grid (backward, unnormalized; the
+    // kx = 0 and Nyquist planes are Hermitian after fim_symmetry)
     if (!m.plans || m.plan_n[0] != n1 || m.plan_n[1] != n2 || m.plan_n[2] != n3 || m.plan_real != int(sizeof(Real))) {
         m.drop_plans();
-        if (n3 > 1) {
-            int n[2] = {n3, n2};
-            if (!fft_ok(hipfftPlanMany(&m.plan_yz, 2, n, n, h, 1, n, h, 1, FftTypes<Real>::c2c, h), "hipfftPlanMany (y, z)"))
-                return false;
-        } else {
-            int n[1] = {n2};
-            if (!fft_ok(hipfftPlanMany(&m.plan_yz, 1, n, n, h, 1, n, h, 1, FftTypes<Real>::c2c, h), "hipfftPlanMany (y)"))
-                return false;
-        }
-        int nx[1] = {n1}, in[1] = {h}, on[1] = {n1};
-        if (!fft_ok(hipfftPlanMany(&m.plan_x, 1, nx, in, 1, h, on, 1, n1, FftTypes<Real>::c2r, int(lines)),
-                    "hipfftPlanMany (x)")) {
-            (void)hipfftDestroy(m.plan_yz);
-            return false;
-        }
+        const hipfftResult r = n3 > 1 ? hipfftPlan3d(&m.plan, n3, n2, n1, FftTypes<Real>::c2r)
+                                      : hipfftPlan2d(&m.plan, n2, n1, FftTypes<Real>::c2r);
+        if (!fft_ok(r, "hipfftPlan (c2r)")) return false;
         m.plans = true;
         m.plan_n[0] = n1;
         m.plan_n[1] = n2;
         m.plan_n[2] = n3;
         m.plan_real = int(sizeof(Real));
     }
-    if (!fft_ok(hipfftSetStream(m.plan_yz, stream), "hipfftSetStream") ||
-        !fft_ok(hipfftSetStream(m.plan_x, stream), "hipfftSetStream"))
-        return false;
+    if (!fft_ok(hipfftSetStream(m.plan, stream), "hipfftSetStream")) return false;
 
     Cplx<Real> *S = static_cast<Cplx<Real> *>(m.d_spec);
     Real *R = static_cast<Real *>(m.d_real);
@@ -487,36 +524,45 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
     fim_symmetry<Real><<<static_cast<int>((sym + 255) / 256), 256, 0, stream>>>(S, n1, n2, n3, h);
     if (!ok(hipGetLastError(), "fim_symmetry launch")) return false;
     if (!ok(hipEventRecord(m.ev[2], stream), "hipEventRecord")) return false;
-    if (!fft_ok(FftTypes<Real>::exec_c2c(m.plan_yz, S), "hipfftExec (y, z)")) return false;
-    fim_dc_real<Real><<<static_cast<int>((2 * lines + 255) / 256), 256, 0, stream>>>(S, n1, h, lines);
-    if (!ok(hipGetLastError(), "fim_dc_real launch")) return false;
-    if (!fft_ok(FftTypes<Real>::exec_c2r(m.plan_x, S, R), "hipfftExec (x)")) return false;
+    if (!fft_ok(FftTypes<Real>::exec_c2r(m.plan, S, R), "hipfftExecC2R")) return false;
     if (!ok(hipEventRecord(m.ev[3], stream), "hipEventRecord")) return false;
-    // scaling, statistics, rescale and taper on the internal grid
-    fim_scale<Real><<<kScaleBlocks, 256, 0, stream>>>(R, total, s.scaling, m.d_part);
-    if (!ok(hipGetLastError(), "fim_scale launch")) return false;
+    // statistics of the scaled internal field; scaling, rescale and taper
+    fim_moments<Real><<<kScaleBlocks, 256, 0, stream>>>(R, total, s.scaling, m.d_part);
+    if (!ok(hipGetLastError(), "fim_moments launch")) return false;
     fim_finalize<Real><<<1, 256, 0, stream>>>(m.d_part, kScaleBlocks, static_cast<double>(s.sigma), m.d_stat);
     if (!ok(hipGetLastError(), "fim_finalize launch")) return false;
+    Real scale = s.scaling;  // still to apply to R when the mapping reads it
     if (rescale == 1 || npoi > 0) {
-        const int eblocks = static_cast<int>(std::min<long>((total + 255) / 256, 8192));
-        fim_epilogue<Real><<<eblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, m.d_stat, rescale == 1 ? 1 : 0,
-                                                        static_cast<const Real *>(m.d_pt), npoi,
-                                                        static_cast<Real>(mute), static_cast<Real>(taper));
+        const int blocks = static_cast<int>(std::min<long>(lines, kRowBlocks));
+        fim_epilogue<Real><<<blocks, 256, 0, stream>>>(R, n1, n2, static_cast<int>(lines), s.scaling, s.dh, m.d_stat,
+                                                       rescale == 1 ? 1 : 0, static_cast<const Real *>(m.d_pt),
+                                                       npoi, static_cast<Real>(mute), static_cast<Real>(taper));
         if (!ok(hipGetLastError(), "fim_epilogue launch")) return false;
+        scale = Real(1);
     }
     if (!ok(hipEventRecord(m.ev[4], stream), "hipEventRecord")) return false;
     // nearest-neighbour values at the caller's points
     if (structured) {
         const int nx = fe[0] - fs[0] + 1, ny = fe[1] - fs[1] + 1, nz = fe[2] - fs[2] + 1;
-        const long nout = long(nx) * ny * nz;
-        const int mblocks = static_cast<int>(std::max<long>(1, std::min<long>((nout + 255) / 256, 8192)));
-        fim_map_grid<Real><<<mblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, d_field, nx, ny, nz, fs[0], fs[1], fs[2],
-                                                        s.ds, s.off_axis[0], s.off_axis[1], s.off_axis[2]);
+        const long rows = long(ny) * nz;
+        if (rows >= (long(1) << 31)) {
+            err = "FIM: output grid too large for the mapping launch";
+            return false;
+        }
+        const int blocks = static_cast<int>(std::min<long>(rows, kRowBlocks));
+        if (!m.grow(reinterpret_cast<void **>(&m.d_tab), m.tab_cap, sizeof(int) * size_t(nx + ny + nz), err))
+            return false;
+        fim_axis_tables<Real><<<(nx + ny + nz + 255) / 256, 256, 0, stream>>>(
+            m.d_tab, nx, ny, nz, fs[0], fs[1], fs[2], s.ds, s.dh, s.off_axis[0], s.off_axis[1], s.off_axis[2], n1, n2,
+            n3);
+        if (!ok(hipGetLastError(), "fim_axis_tables launch")) return false;
+        fim_map_grid<Real><<<blocks, 256, 0, stream>>>(R, n1, n2, n3, scale, d_field, m.d_tab, nx, ny,
+                                                       static_cast<int>(rows));
     } else {
         const int mblocks = static_cast<int>(std::max<long>(1, std::min<long>((npoints + 255) / 256, 8192)));
-        fim_map_points<Real><<<mblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, d_field, d_points, d_points + npoints,
-                                                          d_points + 2 * npoints, npoints, s.off_axis[0],
-                                                          s.off_axis[1], s.off_axis[2]);
+        fim_map_points<Real><<<mblocks, 256, 0, stream>>>(R, n1, n2, n3, s.dh, scale, d_field, d_points,
+                                                          d_points + npoints, d_points + 2 * npoints, npoints,
+                                                          s.off_axis[0], s.off_axis[1], s.off_axis[2]);
     }
     if (!ok(hipGetLastError(), "fim_map launch")) return false;
     if (!ok(hipEventRecord(m.ev[5], stream), "hipEventRecord")) return false;

@@ -477,7 +477,7 @@ bool Device::ensure(std::string &err)
     int count = 0;
     const hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count < 1) {
-        err = "no HIP device available (the SRM runs only on the GPU; there is no CPU fallback)";
+        err = "no HIP device available (SRM and FIM run only on the GPU; there is no CPU fallback)";
         return false;
     }
     HIP_TRY(hipGetDevice(&device_));

@@ -9,10 +9,14 @@
  *      Karman, cl = 2000 m, sigma = 0.05, Hurst 0.25, seed 1235), rotated by
  *      alpha = 30, beta = -10, with two points of interest, mute 500 m and
  *      taper 2500 m;
- *   2. the same grid points given as a point list (scarf_nocart_initialize).
+ *   2. the same grid points given as a point list (scarf_nocart_initialize);
+ *   3. FIM on the grid as the example's own 3D Cartesian FIM block runs it
+ *      (examples/c/driver.c:405-430: method left at its default 0, hurst
+ *      the only option).
  *
- * Each field goes to <outdir>/c_cart.bin and <outdir>/c_nocart.bin through
- * scarf_io_one (the point list as raw values); a z slice through
+ * Each field goes to <outdir>/c_cart.bin, <outdir>/c_nocart.bin and
+ * <outdir>/c_fim_cart.bin through scarf_io_one (the point list as raw
+ * values); a z slice through
  * scarf_io_slice; stats(1:6) are printed as
  * "stats <name> s0 .. s5". Usage: srm_driver nx ny nz outdir
  */
@@ -103,6 +107,17 @@ int main(int argc, char **argv)
     FILE *f = fopen(path, "wb");
     if (!f || fwrite(field, sizeof(real), (size_t)npts, f) != (size_t)npts) return 1;
     fclose(f);
+
+    /* 3. FIM (examples/c/driver.c:405-430) */
+    scarf_opt_init(&options);
+    options.hurst = hurst;
+    scarf_cart_initialize(nd, fs, fe, ds, acf, cl, sigma, &options);
+    memset(field, 0, sizeof(real) * npts);
+    scarf_execute(seed, field, stats);
+    print_stats("fim_cart", stats);
+    snprintf(path, sizeof path, "%s/c_fim_cart.bin", outdir);
+    scarf_io_one(nd, n, field, path, NULL);
+    scarf_finalize();
 
     free(field);
     free(x);

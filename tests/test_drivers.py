@@ -1,7 +1,8 @@
 """The C and C++ drop-in drivers (tests/c/srm_driver.c,
 tests/cpp/srm_driver.cpp): callers written against include/scarf3d.h the
 way the reference's examples are (examples/c/driver.c:526-612 with
-method = 1; examples/c++/driver.cpp with Scarf3D::Initialize<srm>), built by
+method = 1 and its FIM block :405-430; examples/c++/driver.cpp with
+Scarf3D::Initialize<srm>), built by
 scarf3d_amd/build.py for both library flavours. On the GPU every field they
 write equals the ctypes path's on the same parameters, bit for bit, and
 their printed statistics equal its stats(1:4)."""
@@ -50,9 +51,9 @@ def test_drivers_are_built_and_report_errors(kind, flavour, tmp_path):
     assert "no HIP device available" in res.stderr
 
 
-def _lib_field(lib, nd, opts, points=None):
+def _lib_field(lib, nd, opts, points=None, method=1):
     assert lib.set_kernel("auto") == 0  # as in the drivers' own processes
-    o = lib.opt(method=1, **opts)
+    o = lib.opt(method=method, **opts)
     if points is None:
         n = N[:nd]
         lib.cart_initialize(nd, [1] * nd, list(n), DS, 0, list(CL[:nd]), SIGMA, o)
@@ -77,6 +78,7 @@ def test_drivers_equal_ctypes_path(lib_f, lib_d, flavour, tmp_path):
         "c_cart": _lib_field(lib, 3, dict(hurst=HURST, alpha=30.0, beta=-10.0, npoi=2,
                                               poi=poi.reshape(-1, order="F"), taper=2500.0, mute=500.0)),
         "c_nocart": _lib_field(lib, 3, dict(hurst=HURST), points=pts),
+        "c_fim_cart": _lib_field(lib, 3, dict(hurst=HURST), method=0),
         "cpp_2d": _lib_field(lib, 2, dict(hurst=HURST)),
         "cpp_3d": _lib_field(lib, 3, dict(hurst=HURST, alpha=30.0, npoi=2, poi=poi.reshape(-1, order="F"),
                                               taper=2500.0, mute=500.0)),

@@ -238,3 +238,37 @@ def test_realization_is_seeded_and_rescaled():
     R = a["internal"]
     k = np.argwhere(R == 0.0)
     assert len(k) > 0
+
+
+def _hermitian_line_rule(S: np.ndarray, n1: int) -> np.ndarray:
+    """The library's symmetry pass (fim_symmetry): the reference's rules, with
+    the 3D k = 0 line's pairs (j, n2 - j) replaced by their Hermitian part."""
+    out = fo.enforce_symmetry(S, n1)
+    n3, n2, _ = S.shape
+    if n3 > 1:
+        for p in (0, n1 // 2):
+            P = S[:, :, p].copy()  # values after the point conditions on k = 0, 0 < j < n2 (untouched there)
+            for j in range(1, n2 // 2):
+                v = (P[0, j] + P[0, n2 - j]) / 2
+                out[0, j, p] = v
+                out[0, n2 - j, p] = np.conj(v)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 10), (10, 12, 4), (6, 8, 2), (12, 10, 1)])
+def test_hermitian_planes_with_one_c2r_equal_reference_sequence(shape):
+    """One 3D c2r of the spectrum whose kx = 0 / Nyquist planes carry the
+    Hermitian part of the in-place line equals the reference's z, y transforms
+    followed by a c2r along x that reads real parts only."""
+    n1, n2, n3 = shape
+    rng = np.random.default_rng(sum(shape))
+    h = n1 // 2 + 1
+    S = rng.normal(size=(n3, n2, h)) + 1j * rng.normal(size=(n3, n2, h))
+    ref = fo.transform(fo.enforce_symmetry(S, n1), n1)
+    H = _hermitian_line_rule(S, n1)
+    for p in (0, n1 // 2):  # the planes are exactly Hermitian now
+        P = H[:, :, p]
+        mirror = np.conj(P[(-np.arange(n3)) % n3][:, (-np.arange(n2)) % n2])
+        np.testing.assert_array_equal(P, mirror)
+    got = np.fft.irfftn(H, s=(n3, n2, n1), axes=(0, 1, 2), norm="forward")
+    np.testing.assert_allclose(got, ref, atol=1e-12)
