@@ -186,7 +186,8 @@ int main(int argc, char **argv)
     }
     scarf_cart_initialize(in.ndim, fs, fe, static_cast<real>(in.ds), in.acf, cl, static_cast<real>(in.sigma), &opt);
     if (*scarf_last_error()) return 1;
-    // present optional arguments of scarf_initialize (standalone/main.f90:426-427)
+    // present optional arguments of scarf_initialize (standalone/main.f90:426-427);
+    // rescale and pad are read and checked but, as there, passed to no call
     const struct {
         const char *name;
         double value;

@@ -88,3 +88,27 @@ def test_standalone_field_equals_library(tmp_path, lib_f):
     assert lib_f.set_option("mute", 0.0) == 0  # present, as the standalone passes it (a C opt drops mute <= 0)
     ref, _ = lib_f.execute(1234)
     assert np.array_equal(field, ref.reshape(-1, order="F"))
+
+
+@pytest.mark.gpu
+def test_standalone_fim_field_equals_library(tmp_path, lib_f):
+    """method 0 (FIM) with a tapered point of interest: the program's field is
+    the library's FIM field on the same options. The input's "# pad 1" is read
+    and passed to no call, as in standalone/main.f90:204-205, :426-427, so the
+    field is the unpadded one."""
+    from scarf3d_amd.workloads import init_library
+
+    text = GOOD.replace("# method\n1\n", "# method\n0\n# pad\n1\n") + "# poi\n500 500 500\n# mute\n0\n# taper\n300\n"
+    print("[case] standalone FIM 40x36x20", flush=True)
+    r = _run(tmp_path, text)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Program completed successfully" in r.stdout
+    field = np.fromfile(tmp_path / "random_field.bin", dtype=np.float32)
+    cfg = structured_cfg(n=(40, 36, 20), ds=50.0, cl=(2000.0,) * 3, sigma=0.05, hurst=0.25,
+                         poi=np.array([[500.0], [500.0], [500.0]]), mute=0.0, taper=300.0, real=np.float32)
+    cfg["method"], cfg["pad"] = 0, 0
+    init_library(lib_f, cfg)
+    assert lib_f.set_option("mute", 0.0) == 0  # present, as the standalone passes it
+    ref, _ = lib_f.execute(1234)
+    assert lib_f.last_error() == ""
+    assert np.array_equal(field, ref.reshape(-1, order="F"))
