@@ -18,7 +18,11 @@ library, and rounded to float through the float library): every kernel
 against the FP64 kernel at every point, and against the oracle at 256
 sampled points.
 
-The 64-member ensemble: every member equal to its single run bit for bit."""
+The 64-member ensemble: every member equal to its single run bit for bit.
+
+FIM (method 0) on the 1024 x 1024 x 512 grid with padding, rescale and taper:
+z-slab blocks equal to the whole field bit for bit, and the rescale and mute
+invariants."""
 import numpy as np
 import pytest
 
@@ -208,3 +212,51 @@ def test_ensemble_64_full_size(lib_f):
     print(f"  {len(members)} members equal their single runs", flush=True)
     for b in bufs:
         b.free()
+
+
+def test_fim_large_grid_full_size(lib_f):
+    """FIM (method 0) on the 1024 x 1024 x 512 grid with padding, rescale and the
+    16 tapered points of interest (internal grid 1626 x 1426 x 664 with the
+    padding of cl/dh points per side; 271 and 83 are prime, so rocFFT takes
+    Bluestein's algorithm). Size-independent checks, the oracle
+    being covered on smaller grids (tests/test_gpu_fim.py):
+      - four z-slab blocks computed separately (the model's nc/fc, as MPI
+        ranks of the reference pass them) equal the whole field bit for bit;
+      - the rescale reports sigma and zero mean, and the field's values
+        outside the tapered zones carry the rescaled spread: their standard
+        deviation is sigma within sampling (the internal grid is larger than
+        the output block);
+      - the points of interest are muted: the internal node behind the output
+        point nearest each one lies within 2 x 10 sqrt(3) m of it, where the
+        taper factor (mute 20 m, taper 500 m) is below 0.0022, so that value is
+        below 0.02 sigma.
+    """
+    from scarf3d_amd.workloads import WORKLOADS, init_library, workload_cfg
+
+    cfg = workload_cfg(WORKLOADS["large_1024"])
+    cfg["method"], cfg["pad"] = 0, 1
+    n = tuple(cfg["fe"])
+    print("[case] FIM 1024 x 1024 x 512, pad, rescale, 16 POIs", flush=True)
+    init_library(lib_f, cfg)
+    whole, st = lib_f.execute(77)
+    assert lib_f.last_error() == "", lib_f.last_error()
+    lib_f.finalize()
+    assert float(st[2]) == pytest.approx(cfg["sigma"], rel=1e-6) and st[3] == 0.0
+    sample = whole[::7, ::5, ::3].astype(np.float64)
+    assert abs(sample.std() / cfg["sigma"] - 1.0) < 0.25
+    poi = np.asarray(cfg["poi"])
+    for q in range(poi.shape[1]):
+        i, j, k = (int(round(poi[d, q] / cfg["ds"])) for d in range(3))
+        assert abs(float(whole[min(i, n[0] - 1), min(j, n[1] - 1), min(k, n[2] - 1)])) <= 0.02 * cfg["sigma"], q
+    for b in range(4):
+        z0 = b * (n[2] // 4)
+        c = dict(cfg)
+        c["fs"] = (1, 1, z0 + 1)
+        c["fe"] = (n[0], n[1], z0 + n[2] // 4)
+        print(f"[case] FIM block z {z0 + 1}..{z0 + n[2] // 4}", flush=True)
+        init_library(lib_f, c)
+        part, stb = lib_f.execute(77)
+        assert lib_f.last_error() == "", lib_f.last_error()
+        lib_f.finalize()
+        assert np.array_equal(part, whole[:, :, z0:z0 + n[2] // 4]), b
+        assert np.array_equal(stb[:4], st[:4]), b
