@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -392,10 +393,22 @@ struct FimDevice::Impl {
     hipEvent_t ev[6] = {};
     size_t reserved = 0;
 
-    bool grow(void **p, size_t &cap, size_t bytes, std::string &err)
+    // grows *p to `bytes` (kept when large enough); the stream is drained
+    // before a buffer it may still use is freed
+    bool grow(void **p, size_t &cap, size_t bytes, hipStream_t stream, size_t elsewhere, std::string &err)
     {
-        if (bytes <= cap) return true;
+        if (bytes <= cap && *p) return true;
+        if (const char *e = std::getenv("SCARF_DEVICE_MEMORY_LIMIT")) {
+            const double limit = std::atof(e);
+            const size_t after = elsewhere + reserved + bytes - std::min(bytes, cap);
+            if (limit > 0 && double(after) > limit) {
+                err = "device memory limit SCARF_DEVICE_MEMORY_LIMIT=" + std::string(e) + " bytes: the library holds " +
+                      std::to_string(elsewhere + reserved) + " and would need " + std::to_string(after);
+                return false;
+            }
+        }
         if (*p) {
+            (void)hipStreamSynchronize(stream);
             (void)hipFree(*p);
             reserved -= cap;
             *p = nullptr;
@@ -433,7 +446,7 @@ size_t FimDevice::reserved_bytes() const { return impl_ ? impl_->reserved : 0; }
 template <class Real>
 bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double taper, int seed, int fs[3], int fe[3],
                     bool structured, const double *d_points, long npoints, Real *d_field, double stats[4],
-                    hipStream_t stream, std::string &err)
+                    hipStream_t stream, size_t reserved_elsewhere, std::string &err)
 {
     Impl &m = *impl_;
     auto ok = [&](hipError_t e, const char *what) {
@@ -468,11 +481,13 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
         for (auto &e : m.ev)
             if (!ok(hipEventCreate(&e), "hipEventCreate")) return false;
     }
-    if (!m.grow(&m.d_spec, m.spec_cap, sizeof(Cplx<Real>) * size_t(h) * size_t(lines), err)) return false;
-    if (!m.grow(&m.d_real, m.real_cap, sizeof(Real) * size_t(total), err)) return false;
+    if (!m.grow(&m.d_spec, m.spec_cap, sizeof(Cplx<Real>) * size_t(h) * size_t(lines), stream, reserved_elsewhere,
+                err))
+        return false;
+    if (!m.grow(&m.d_real, m.real_cap, sizeof(Real) * size_t(total), stream, reserved_elsewhere, err)) return false;
     const int npoi = static_cast<int>(s.pt.size() / 3);
     if (npoi > 0) {
-        if (!m.grow(&m.d_pt, m.pt_cap, sizeof(Real) * s.pt.size(), err)) return false;
+        if (!m.grow(&m.d_pt, m.pt_cap, sizeof(Real) * s.pt.size(), stream, reserved_elsewhere, err)) return false;
         if (!ok(hipMemcpyAsync(m.d_pt, s.pt.data(), sizeof(Real) * s.pt.size(), hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync"))
             return false;
@@ -550,7 +565,8 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
             return false;
         }
         const int blocks = static_cast<int>(std::min<long>(rows, kRowBlocks));
-        if (!m.grow(reinterpret_cast<void **>(&m.d_tab), m.tab_cap, sizeof(int) * size_t(nx + ny + nz), err))
+        if (!m.grow(reinterpret_cast<void **>(&m.d_tab), m.tab_cap, sizeof(int) * size_t(nx + ny + nz), stream,
+                    reserved_elsewhere, err))
             return false;
         fim_axis_tables<Real><<<(nx + ny + nz + 255) / 256, 256, 0, stream>>>(
             m.d_tab, nx, ny, nz, fs[0], fs[1], fs[2], s.ds, s.dh, s.off_axis[0], s.off_axis[1], s.off_axis[2], n1, n2,
@@ -591,8 +607,8 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
 }
 
 template bool FimDevice::run<float>(const FimSetup<float> &, int, double, double, int, int[3], int[3], bool,
-                                    const double *, long, float *, double[4], hipStream_t, std::string &);
+                                    const double *, long, float *, double[4], hipStream_t, size_t, std::string &);
 template bool FimDevice::run<double>(const FimSetup<double> &, int, double, double, int, int[3], int[3], bool,
-                                     const double *, long, double *, double[4], hipStream_t, std::string &);
+                                     const double *, long, double *, double[4], hipStream_t, size_t, std::string &);
 
 }  // namespace scarf

@@ -68,11 +68,14 @@ public:
     // block (g.structured) or point list (points: FP64 device coordinates,
     // 3 x npts, z zero in 2D). stats[0..3] = {info3, info4} as the
     // reference reports them after an optional rescale, then FP64 mean and
-    // variance of the internal field. Runs on `stream`.
+    // variance of the internal field. Runs on `stream`. reserved_elsewhere:
+    // device bytes the library holds outside FIM, counted against
+    // SCARF_DEVICE_MEMORY_LIMIT=<bytes> when that is set (a buffer that would
+    // take the total above it fails as a failed hipMalloc does).
     template <class Real>
     bool run(const FimSetup<Real> &s, int rescale, double mute, double taper, int seed, int fs[3], int fe[3],
              bool structured, const double *d_points, long npoints, Real *d_field, double stats[4],
-             hipStream_t stream, std::string &err);
+             hipStream_t stream, size_t reserved_elsewhere, std::string &err);
 
     const FimTimings &timings() const { return timings_; }
     // device bytes held by the FIM buffers

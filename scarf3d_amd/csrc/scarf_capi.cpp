@@ -226,6 +226,7 @@ bool ensure_device()
         fail(err);
         return false;
     }
+    s.device->set_external_reserved(s.fim ? s.fim->reserved_bytes() : 0);
     return true;
 }
 
@@ -324,7 +325,7 @@ int execute_fim(int seed, real *d_field, real stats[], const scarf::Device::Slab
     }
     double st[4] = {};
     if (!s.fim->run<real>(fs, s.cfg.rescale, s.cfg.mute, s.cfg.taper, seed, fsi, fei, s.cfg.structured, pts,
-                          s.cfg.npts, d_field, st, s.device->stream(), err))
+                          s.cfg.npts, d_field, st, s.device->stream(), s.device->reserved_bytes(), err))
         return fail(err);
     if (stats) {
         clear_stats(stats);

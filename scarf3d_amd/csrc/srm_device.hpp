@@ -139,6 +139,9 @@ public:
 
     // device bytes held by the library's grow-only buffers
     size_t reserved_bytes() const { return reserved_; }
+    // device bytes the library holds in other buffers (FIM's), counted with
+    // these against SCARF_DEVICE_MEMORY_LIMIT
+    void set_external_reserved(size_t bytes) { external_ = bytes; }
 
 private:
     // Grows *ptr to `bytes` (keeps it when large enough). With
@@ -157,6 +160,7 @@ private:
         return ok;
     }
     size_t reserved_ = 0;
+    size_t external_ = 0;
     // statistics, epilogue, field hand-over and timings after the field
     // kernels; slab_first/slab_points name the parts whose events are
     // slab_ev_. part/stat/poi: device room for the partials, the statistics

@@ -496,10 +496,10 @@ bool Device::reserve(void **ptr, size_t &cap, size_t bytes, std::string &err)
     const bool grow = !(bytes <= cap && *ptr);
     if (const char *e = std::getenv("SCARF_DEVICE_MEMORY_LIMIT")) {
         const double limit = std::atof(e);
-        const size_t after = reserved_ + (grow ? bytes - std::min(bytes, cap) : 0);
+        const size_t after = reserved_ + external_ + (grow ? bytes - std::min(bytes, cap) : 0);
         if (limit > 0 && double(after) > limit) {
             err = "device memory limit SCARF_DEVICE_MEMORY_LIMIT=" + std::string(e) + " bytes: the library holds " +
-                  std::to_string(reserved_) + " and would need " + std::to_string(after);
+                  std::to_string(reserved_ + external_) + " and would need " + std::to_string(after);
             return false;
         }
     }

@@ -152,3 +152,22 @@ def test_fim_blocks_equal_whole_model(lib_f):
         lib_f.finalize()
         assert np.array_equal(f, f_all[:, :, z0:z0 + 48]), b
         assert np.array_equal(st[:4], st_all[:4]), b
+
+
+def test_fim_device_memory_limit_reports_and_recovers(lib_f, monkeypatch):
+    """SCARF_DEVICE_MEMORY_LIMIT below what FIM's buffers need: execute fails
+    with the limit's error and zero statistics, then runs normally without it."""
+    cfg = _fim(structured_cfg(n=(40, 36, 20), ds=50.0, cl=(500.0, 400.0, 300.0), hurst=0.25, real=np.float32), 1)
+    ref, st_ref = _run(lib_f, cfg, 3)
+    lib_f.finalize()
+    monkeypatch.setenv("SCARF_DEVICE_MEMORY_LIMIT", "65536")
+    _say("FIM, device memory limit 64 KiB")
+    big = _fim(structured_cfg(n=(80, 72, 40), ds=50.0, cl=(500.0, 400.0, 300.0), hurst=0.25, real=np.float32), 1)
+    init_library(lib_f, big)  # a larger grid: the field and FIM's buffers must grow
+    field, stats = lib_f.execute(3)
+    assert "SCARF_DEVICE_MEMORY_LIMIT" in lib_f.last_error()
+    assert not np.any(stats)
+    lib_f.finalize()
+    monkeypatch.delenv("SCARF_DEVICE_MEMORY_LIMIT")
+    field, stats = _run(lib_f, cfg, 3)
+    assert np.array_equal(field, ref) and np.array_equal(stats[:4], st_ref[:4])
