@@ -87,7 +87,12 @@ __global__ __launch_bounds__(256) void fim_spectrum(Cplx<Real> *__restrict__ S, 
             const Real fn = a.acf == 0 ? Real(1) / pow(Real(1) + kl * kl, a.nu) : exp(Real(-0.25) * (kl * kl));
             const Real amp = sqrt(a.cnst * fn) * butter;
             const Real ph = (u * Real(2)) * pi;
-            const Real c = cos(ph), sn = sin(ph);
+            Real c, sn;
+            if constexpr (sizeof(Real) == sizeof(float)) {
+                sincosf(ph, &sn, &c);
+            } else {
+                sincos(ph, &sn, &c);
+            }
             S[size_t(i) + size_t(a.h) * size_t(line)] = Cplx<Real>{c * amp, sn * amp};
         });
     }
@@ -391,6 +396,7 @@ struct FimDevice::Impl {
     int plan_n[3] = {0, 0, 0};
     int plan_real = 0;  // sizeof(Real) of the plans
     hipEvent_t ev[6] = {};
+    bool ready = false;  // the fixed buffers and events exist
     size_t reserved = 0;
 
     // grows *p to `bytes` (kept when large enough); the stream is drained
@@ -416,6 +422,7 @@ struct FimDevice::Impl {
         }
         const hipError_t e = hipMalloc(p, bytes);
         if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation leaves no error for the next call
             *p = nullptr;
             err = std::string("FIM: hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e);
             return false;
@@ -451,6 +458,7 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
     Impl &m = *impl_;
     auto ok = [&](hipError_t e, const char *what) {
         if (e == hipSuccess) return true;
+        (void)hipGetLastError();  // leave no error behind for the launch checks of a later call
         err = std::string("FIM: ") + what + ": " + hipGetErrorString(e);
         return false;
     };
@@ -466,20 +474,36 @@ bool FimDevice::run(const FimSetup<Real> &s, int rescale, double mute, double ta
         err = "FIM: internal grid too large";
         return false;
     }
-    if (!m.d_gen) {
-        GenTables t;
-        fim_stream::make_tables(t);
-        if (!ok(hipMalloc(&m.d_gen, sizeof(GenTables)), "hipMalloc")) return false;
-        m.reserved += sizeof(GenTables);
-        if (!ok(hipMemcpy(m.d_gen, &t, sizeof t, hipMemcpyHostToDevice), "hipMemcpy")) return false;
-        if (!ok(hipMalloc(&m.d_part, sizeof(double) * 3 * kScaleBlocks), "hipMalloc")) return false;
-        if (!ok(hipMalloc(&m.d_stat, sizeof(double) * 8), "hipMalloc")) return false;
-        m.reserved += sizeof(double) * (3 * kScaleBlocks + 8);
-        if (!ok(hipHostMalloc(reinterpret_cast<void **>(&m.h_stat), sizeof(double) * 8, hipHostMallocDefault),
-                "hipHostMalloc"))
+    // fixed buffers, made once (each on its own, so that a failure leaves the
+    // rest to the next call)
+    if (!m.ready) {
+        if (!m.d_gen) {
+            GenTables t;
+            fim_stream::make_tables(t);
+            void *g = nullptr;
+            if (!ok(hipMalloc(&g, sizeof(GenTables)), "hipMalloc")) return false;
+            if (!ok(hipMemcpy(g, &t, sizeof t, hipMemcpyHostToDevice), "hipMemcpy")) {
+                (void)hipFree(g);
+                return false;
+            }
+            m.d_gen = static_cast<GenTables *>(g);
+            m.reserved += sizeof(GenTables);
+        }
+        if (!m.d_part) {
+            if (!ok(hipMalloc(&m.d_part, sizeof(double) * 3 * kScaleBlocks), "hipMalloc")) return false;
+            m.reserved += sizeof(double) * 3 * kScaleBlocks;
+        }
+        if (!m.d_stat) {
+            if (!ok(hipMalloc(&m.d_stat, sizeof(double) * 8), "hipMalloc")) return false;
+            m.reserved += sizeof(double) * 8;
+        }
+        if (!m.h_stat && !ok(hipHostMalloc(reinterpret_cast<void **>(&m.h_stat), sizeof(double) * 8,
+                                           hipHostMallocDefault),
+                             "hipHostMalloc"))
             return false;
         for (auto &e : m.ev)
-            if (!ok(hipEventCreate(&e), "hipEventCreate")) return false;
+            if (!e && !ok(hipEventCreate(&e), "hipEventCreate")) return false;
+        m.ready = true;
     }
     if (!m.grow(&m.d_spec, m.spec_cap, sizeof(Cplx<Real>) * size_t(h) * size_t(lines), stream, reserved_elsewhere,
                 err))

@@ -171,3 +171,18 @@ def test_fim_device_memory_limit_reports_and_recovers(lib_f, monkeypatch):
     monkeypatch.delenv("SCARF_DEVICE_MEMORY_LIMIT")
     field, stats = _run(lib_f, cfg, 3)
     assert np.array_equal(field, ref) and np.array_equal(stats[:4], st_ref[:4])
+
+
+def test_fim_points_double_library(lib_d):
+    cfg = _fim(unstructured_cfg(npts=2500, box=(3000.0, 2500.0, 2000.0), dh=50.0, acf=1, cl=(400.0, 300.0, 300.0),
+                                sigma=0.1, seed=5, real=np.float64, nd=3), 0)
+    _say("points 3D, libscarf3d")
+    _check(lib_d, cfg, 12, "d")
+
+
+def test_fim_points_2d_rescale_taper(lib_f):
+    cfg = _fim(unstructured_cfg(npts=4000, box=(5000.0, 4000.0, 0.0), dh=40.0, acf=0, hurst=0.4,
+                                cl=(500.0, 400.0, 0.0), sigma=0.07, seed=9, real=np.float32, nd=2, rescale=1,
+                                poi=np.array([[1500.0, 3500.0], [2000.0, 1000.0]]), mute=80.0, taper=600.0), 1)
+    _say("points 2D, rescale, two tapered points")
+    _check(lib_f, cfg, 13, "f")
