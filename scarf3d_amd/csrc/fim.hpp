@@ -10,8 +10,11 @@
 // MPI rank of the reference does, and every step is a device pass:
 //   fim_spectrum   uniforms of the global stream + amplitude, half spectrum
 //   fim_symmetry   the conjugate pairs of the kx = 0 and Nyquist planes
-//   hipFFT         backward c2c over (y, z), then c2r along x
-//   fim_scale      scaling + FP64 moments; finalize; rescale + taper
+//                  (after it both planes are exactly Hermitian)
+//   hipFFT         one 3D (2D) c2r, equal to the reference's backward c2c
+//                  along z and y followed by c2r along x
+//   fim_moments    FP64 moments of the scaled field; fim_finalize
+//   fim_epilogue   scaling, optional rescale and taper on the internal grid
 //   fim_map_*      nearest-neighbour values at the caller's points
 // Host-side constants (internal grid size, offsets, spectrum constants) are
 // computed at the library's `real` precision with the reference's roundings
