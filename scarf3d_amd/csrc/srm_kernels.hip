@@ -59,7 +59,9 @@ __global__ __launch_bounds__(256) void stats_finalize(const double *__restrict__
     }
     if (threadIdx.x == 0) {
         const double n = sm[0][0], mean = sm[1][0], m2 = sm[2][0];
-        const double var = n > 1.0 ? m2 / (n - 1.0) : 0.0;
+        // n - 1 divisor as the reference's variance (scarflib_common.f90:684):
+        // a one-point field gives 0/0 = NaN there and here
+        const double var = m2 / (n - 1.0);
         const Real info3 = sqrt(static_cast<Real>(var));
         const Real info4 = static_cast<Real>(mean);
         const Real sc = static_cast<Real>(sigma) / info3;

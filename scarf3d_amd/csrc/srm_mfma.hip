@@ -677,7 +677,9 @@ __global__ __launch_bounds__(256) void ensemble_finalize(const MemberDesc *__res
     }
     if (threadIdx.x == 0) {
         const double n = sm[0][0], mean = sm[1][0], m2 = sm[2][0];
-        const double var = n > 1.0 ? m2 / (n - 1.0) : 0.0;
+        // n - 1 divisor as the reference's variance (scarflib_common.f90:684):
+        // a one-point field gives 0/0 = NaN there and here
+        const double var = m2 / (n - 1.0);
         const Real info3 = sqrt(static_cast<Real>(var));
         double *o = stat + 6 * blockIdx.x;
         o[0] = n;

@@ -89,6 +89,15 @@ def init_library(lib, cfg: dict) -> None:
     else:
         if cfg["ds"] != cfg["dh"]:
             opts["ds"] = cfg["ds"]
+        # model extents other than the points' own min/max (the default of
+        # src/scarflib.f90:264-271): passed as nc/fc
+        r = lib.dtype
+        coords = [cfg["x"], cfg["y"], cfg["z"]][:nd]
+        own_nc = [float(np.asarray(c, dtype=r).min()) for c in coords]
+        own_fc = [float(np.asarray(c, dtype=r).max()) for c in coords]
+        if list(cfg["nc"][:nd]) != own_nc or list(cfg["fc"][:nd]) != own_fc:
+            opts["nc"] = list(cfg["nc"][:nd])
+            opts["fc"] = list(cfg["fc"][:nd])
         o = lib.opt(**opts)
         lib.nocart_initialize(nd, cfg["x"], cfg["y"], cfg["z"] if nd == 3 else None, cfg["dh"], cfg["acf"],
                               cfg["cl"][:nd], cfg["sigma"], o)

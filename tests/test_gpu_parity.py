@@ -264,3 +264,66 @@ def test_device_memory_limit_reports_and_recovers(lib_f, monkeypatch):
     monkeypatch.delenv("SCARF_DEVICE_MEMORY_LIMIT")
     field, stats = _run(lib_f, cfg, 21, "mfma_bf16x3")
     assert np.array_equal(field, ref) and np.array_equal(stats[:4], st_ref[:4])
+
+
+def _block(n, fs, model, ds=50.0, **kw):
+    """Block n from global index fs of a model of `model` points (nc/fc of the
+    whole model, as each MPI rank of the reference passes them)."""
+    cfg = structured_cfg(n=n, fs=fs, ds=ds, real=np.float32, **kw)
+    cfg["nc"] = (0.0, 0.0, 0.0)
+    cfg["fc"] = tuple(float(np.float32((m - 1) * ds)) for m in model)
+    return cfg
+
+
+EDGE_CASES = [
+    ("one_point_block", lambda: _block((1, 1, 1), (7, 5, 3), (40, 30, 20), nharm=200)),
+    ("one_x_plane", lambda: _block((1, 33, 9), (4, 1, 1), (12, 33, 9), nharm=176)),
+    ("one_column", lambda: _block((130, 1, 1), (1, 6, 2), (130, 10, 6), nharm=208)),
+    ("one_harmonic", lambda: structured_cfg(n=(20, 18, 10), nharm=1, real=np.float32)),
+    ("odd_harmonic_count", lambda: structured_cfg(n=(20, 18, 10), nharm=23, real=np.float32)),
+]
+
+
+@pytest.mark.parametrize("kernel", ["direct_f32", "mfma_bf16x3", "mfma_f16x2", "direct_f64"])
+@pytest.mark.parametrize("name,make", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_edge_shapes_match_oracle(oracle, lib_f, name, make, kernel):
+    """Degenerate blocks (one point, one x plane, one column of x), one
+    harmonic and a harmonic count that pads the MFMA stage: field against the
+    oracle, statistics against the oracle pipeline. The variance's n - 1
+    divisor makes stats(3) of a one-point block 0/0 = NaN, in the reference's
+    formula (scarflib_common.f90:684) as in the library."""
+    cfg = make()
+    seed = 4242
+    _say(name, kernel, cfg["fs"], cfg["fe"], "nharm", cfg["nharm"])
+    ref, _ = oracle_field(oracle, cfg, seed, "f")
+    _, ref_stats = oracle.srm(oracle.params(cfg), seed, "f")
+    field, stats = _run(lib_f, cfg, seed, kernel)
+    ref = ref.reshape(field.shape, order="F")
+    err = np.abs(field.astype(np.float64) - ref).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}, stats {stats[:4]} vs {ref_stats[:4]}", flush=True)
+    assert err <= (1e-6 if kernel == "direct_f64" else TOL[kernel])
+    assert stats[0] == ref_stats[0] and stats[1] == ref_stats[1]
+    if field.size == 1:
+        assert np.isnan(ref_stats[2]) and np.isnan(stats[2])
+        assert stats[3] == pytest.approx(ref_stats[3], rel=1e-5, abs=1e-6 * cfg["sigma"])
+    else:
+        assert stats[2] == pytest.approx(ref_stats[2], rel=1e-5, abs=1e-7 * cfg["sigma"])
+        assert abs(stats[3] - ref_stats[3]) <= 1e-5 * cfg["sigma"]
+
+
+@pytest.mark.parametrize("npts", [1, 65])
+def test_short_point_lists_match_oracle(oracle, lib_f, npts):
+    """Point lists of one point and of one 64-point block plus one, inside a
+    model whose nc/fc are given: field and statistics against the oracle."""
+    cfg = unstructured_cfg(npts=npts, nharm=300, seed=3, nc=(0.0, 0.0, 0.0), fc=(10000.0, 10000.0, 5000.0))
+    _say("points", npts)
+    ref, _ = oracle_field(oracle, cfg, 11, "f")
+    _, ref_stats = oracle.srm(oracle.params(cfg), 11, "f")
+    field, stats = _run(lib_f, cfg, 11, "direct_f32")
+    err = np.abs(field.astype(np.float64) - ref).max() / cfg["sigma"]
+    print(f"  max|err|/sigma = {err:.3e}, stats {stats[:4]} vs {ref_stats[:4]}", flush=True)
+    assert err <= TOL["direct_f32"]
+    if npts == 1:
+        assert np.isnan(ref_stats[2]) and np.isnan(stats[2])
+    else:
+        assert stats[2] == pytest.approx(ref_stats[2], rel=1e-5)
