@@ -101,6 +101,14 @@ scarf::WorkerPool &drain_pool()
     return *p;
 }
 
+// ensemble members' host draws: one member per thread at a time
+scarf::WorkerPool &member_pool()
+{
+    static scarf::WorkerPool *p = new scarf::WorkerPool(
+        static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency()))));  // lives until exit
+    return *p;
+}
+
 bool copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t stream, std::string &err)
 {
     auto ok = [&](hipError_t e, const char *what) {
@@ -816,38 +824,44 @@ static int scarf_ensemble_execute_body(int n, const struct scarf_member *members
     std::vector<double> host_s(n, 0.0);
     std::vector<std::string> errs(n);
     std::atomic<int> next{0};
-    auto worker = [&]() {
+    auto draw_member = [&](int m) {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::string e;
+        const int rc = kDouble ? scarf::srm_setup<double>(cfgs[m], setups[m], e)
+                               : scarf::srm_setup<float>(cfgs[m], setups[m], e);
+        if (rc != 0) {
+            errs[m] = e;
+            return;
+        }
+        // members are drawn in parallel already: one thread per member's draw
+        const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed, 1)
+                                              : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed, 1);
+        const scarf::HarmonicTable t = scarf::build_table(setups[m], d);
+        const int np_ = std::max(64, (t.count() + 63) / 64 * 64);
+        std::vector<double> &r = records[m];
+        r.assign(size_t(5) * np_, 0.0);
+        for (int l = 0; l < t.count(); ++l) {
+            r[l] = t.wx[l];
+            r[np_ + l] = t.wy[l];
+            r[2 * size_t(np_) + l] = t.wz[l];
+            r[3 * size_t(np_) + l] = t.phase[l];
+            r[4 * size_t(np_) + l] = t.amp[l];
+        }
+        nps[m] = np_;
+        host_s[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    const std::function<void(int)> worker = [&](int) {
         for (int m = next++; m < n; m = next++) {
-            const auto t0 = std::chrono::steady_clock::now();
-            std::string e;
-            const int rc = kDouble ? scarf::srm_setup<double>(cfgs[m], setups[m], e)
-                                   : scarf::srm_setup<float>(cfgs[m], setups[m], e);
-            if (rc != 0) {
-                errs[m] = e;
-                continue;
+            try {
+                draw_member(m);
+            } catch (const std::exception &ex) {
+                errs[m] = std::string("internal error: ") + ex.what();
+            } catch (...) {
+                errs[m] = "internal error";
             }
-            // members are drawn in parallel already: one thread per member's draw
-            const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed, 1)
-                                                  : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed, 1);
-            const scarf::HarmonicTable t = scarf::build_table(setups[m], d);
-            const int np_ = std::max(64, (t.count() + 63) / 64 * 64);
-            std::vector<double> &r = records[m];
-            r.assign(size_t(5) * np_, 0.0);
-            for (int l = 0; l < t.count(); ++l) {
-                r[l] = t.wx[l];
-                r[np_ + l] = t.wy[l];
-                r[2 * size_t(np_) + l] = t.wz[l];
-                r[3 * size_t(np_) + l] = t.phase[l];
-                r[4 * size_t(np_) + l] = t.amp[l];
-            }
-            nps[m] = np_;
-            host_s[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
     };
-    const int nthreads = std::max(1, std::min<int>(n, std::min<unsigned>(16, std::thread::hardware_concurrency())));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
-    for (auto &t : pool) t.join();
+    member_pool().run(worker);
     for (int m = 0; m < n; ++m)
         if (!errs[m].empty()) return fail("member " + std::to_string(m) + ": " + errs[m]);
     if (!ensure_device()) return -1;

@@ -1,6 +1,7 @@
 // A fixed set of host worker threads that run one job per call, all
 // threads together, the caller being worker 0. Kept for the process by its
-// users, so that a call does not start and join threads.
+// users, so that a call does not start and join threads. A job catches its
+// own exceptions: one that left a worker thread would end the process.
 #pragma once
 #include <condition_variable>
 #include <functional>
@@ -12,19 +13,18 @@ namespace scarf {
 
 class WorkerPool {
 public:
+    // A thread that cannot be started ends the construction with its
+    // exception, after the threads already started are stopped and joined.
     explicit WorkerPool(int n) : n_(n)
     {
-        for (int t = 1; t < n_; ++t) threads_.emplace_back([this, t] { loop(t); });
-    }
-    ~WorkerPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+        try {
+            for (int t = 1; t < n_; ++t) threads_.emplace_back([this, t] { loop(t); });
+        } catch (...) {
+            stop_all();
+            throw;
         }
-        cv_.notify_all();
-        for (auto &t : threads_) t.join();
     }
+    ~WorkerPool() { stop_all(); }
     int size() const { return n_; }
     void run(const std::function<void(int)> &job)
     {
@@ -42,6 +42,16 @@ public:
     }
 
 private:
+    void stop_all()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+        threads_.clear();
+    }
     void loop(int t)
     {
         long seen = 0;
