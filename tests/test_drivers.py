@@ -2,7 +2,7 @@
 tests/cpp/srm_driver.cpp): callers written against include/scarf3d.h the
 way the reference's examples are (examples/c/driver.c:526-612 with
 method = 1 and its FIM block :405-430; examples/c++/driver.cpp with
-Scarf3D::Initialize<srm>), built by
+Scarf3D::Initialize<srm> and Scarf3D::Initialize<fim>), built by
 scarf3d_amd/build.py for both library flavours. On the GPU every field they
 write equals the ctypes path's on the same parameters, bit for bit, and
 their printed statistics equal its stats(1:4)."""
@@ -58,7 +58,8 @@ def _lib_field(lib, nd, opts, points=None, method=1):
         n = N[:nd]
         lib.cart_initialize(nd, [1] * nd, list(n), DS, 0, list(CL[:nd]), SIGMA, o)
     else:
-        lib.nocart_initialize(3, *points, DS, 0, list(CL), SIGMA, o)
+        z = points[2] if nd == 3 else None
+        lib.nocart_initialize(nd, points[0], points[1], z, DS, 0, list(CL[:nd]), SIGMA, o)
     lib.set_nharm(10000)
     field, stats = lib.execute(SEED)
     assert lib.last_error() == "", lib.last_error()
@@ -74,6 +75,8 @@ def test_drivers_equal_ctypes_path(lib_f, lib_d, flavour, tmp_path):
     poi = _poi(real)
     grid = np.meshgrid(*[np.arange(N[a]) * DS for a in range(3)], indexing="ij")
     pts = [g.reshape(-1, order="F").astype(real) for g in grid]
+    grid2 = np.meshgrid(*[np.arange(N[a]) * DS for a in range(2)], indexing="ij")
+    pts2 = [g.reshape(-1, order="F").astype(real) for g in grid2]
     expect = {
         "c_cart": _lib_field(lib, 3, dict(hurst=HURST, alpha=30.0, beta=-10.0, npoi=2,
                                               poi=poi.reshape(-1, order="F"), taper=2500.0, mute=500.0)),
@@ -83,6 +86,9 @@ def test_drivers_equal_ctypes_path(lib_f, lib_d, flavour, tmp_path):
         "cpp_3d": _lib_field(lib, 3, dict(hurst=HURST, alpha=30.0, npoi=2, poi=poi.reshape(-1, order="F"),
                                               taper=2500.0, mute=500.0)),
         "cpp_points": _lib_field(lib, 3, dict(hurst=HURST), points=pts),
+        "cpp_fim_3d": _lib_field(lib, 3, dict(hurst=HURST, alpha=30.0, npoi=2, poi=poi.reshape(-1, order="F"),
+                                                  taper=2500.0, mute=500.0), method=0),
+        "cpp_fim_points2d": _lib_field(lib, 2, dict(hurst=HURST), points=pts2, method=0),
     }
     printed = {}
     for kind in ("c", "cpp"):
