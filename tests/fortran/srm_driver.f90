@@ -1,9 +1,12 @@
 ! Test driver of the Fortran module m_scarflib (modelled on the reference's
-! examples/fortran/driver.f90 SRM blocks, without MPI).
+! examples/fortran/driver.f90 SRM and FIM blocks, without MPI).
 !   srm_driver config        : initialize with keyword options, print the
 !                              parameters the library holds, finalize
 !   srm_driver run <file>    : one structured 3D SRM realization written raw
 !                              to <file>, stats printed
+!   srm_driver fim <file>    : the same grid by FIM (method left at its
+!                              default, 0), padded, rescaled and tapered
+!                              around one point of interest
 PROGRAM srm_driver
   USE, INTRINSIC :: iso_c_binding
   USE m_scarflib
@@ -39,6 +42,19 @@ PROGRAM srm_driver
                           taper=500._f_real, rescale=1, alpha=30._f_real, nc=[-10._f_real, -20._f_real, -30._f_real])
     rc = c_get_config(cfg, 32)
     WRITE(*, '(32ES24.15)') cfg
+    CALL scarf_finalize()
+  ELSE IF (TRIM(mode) == 'fim') THEN
+    CALL get_command_argument(2, fname)
+    CALL scarf_initialize(fs, fe, 50._f_real, 0, cl, 0.05_f_real, hurst=0.25_f_real, poi=poi(:, 1:1), &
+                          mute=100._f_real, taper=250._f_real, rescale=1, pad=1)
+    ALLOCATE(field(fe(1), fe(2), fe(3)))
+    CALL scarf_execute(1234, field, stats)
+    IF (LEN_TRIM(scarf_last_error()) > 0) THEN
+      WRITE(*, *) 'error: ', TRIM(scarf_last_error())
+      STOP 1
+    ENDIF
+    CALL scarf_io(fe, field, TRIM(fname))
+    WRITE(*, '(8ES24.15)') stats
     CALL scarf_finalize()
   ELSE
     CALL get_command_argument(2, fname)

@@ -47,3 +47,25 @@ def test_fortran_execute_equals_c_api(lib_f, tmp_path):
     field_c, stats_c = lib_f.execute(1234)
     assert np.array_equal(field_f, field_c)
     assert np.array_equal(stats_f[:4], stats_c[:4].astype(np.float64))
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_fim_equals_c_api(lib_f, tmp_path):
+    """FIM through the Fortran module (method omitted: its default 0), with
+    pad, rescale and one tapered point of interest, equals the C ABI's field
+    and statistics bit for bit."""
+    out = tmp_path / "fim.bin"
+    print("[case] fortran driver fim", flush=True)
+    res = subprocess.run([str(DRIVER), "fim", str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    stats_f = np.array([float(v) for v in res.stdout.split()])
+    field_f = np.fromfile(out, dtype=np.float32).reshape((40, 36, 20), order="F")
+    poi = np.array([[300.0, 400.0, 200.0]]).T
+    cfg = structured_cfg(n=(40, 36, 20), poi=poi, mute=100.0, taper=250.0, rescale=1)
+    cfg["method"], cfg["pad"] = 0, 1
+    init_product(lib_f, cfg)
+    field_c, stats_c = lib_f.execute(1234)
+    assert lib_f.last_error() == ""
+    assert np.array_equal(field_f, field_c)
+    assert np.array_equal(stats_f[:4], stats_c[:4].astype(np.float64))
