@@ -186,3 +186,40 @@ def test_fim_points_2d_rescale_taper(lib_f):
                                 poi=np.array([[1500.0, 3500.0], [2000.0, 1000.0]]), mute=80.0, taper=600.0), 1)
     _say("points 2D, rescale, two tapered points")
     _check(lib_f, cfg, 13, "f")
+
+
+def test_fim_gaussian_2d_filtered_padded(lib_f):
+    """2D Gaussian spectrum on an internal grid finer than the output grid
+    (dh = ds / 2: the spectrum is cut at pi / ds), padded."""
+    cfg = _fim(structured_cfg(n=(70, 50, 1), ds=30.0, dh=15.0, acf=1, cl=(300.0, 200.0, 0.0), sigma=0.1,
+                              real=np.float32), 1)
+    _say("gauss 2d, dh = ds / 2, pad")
+    _check(lib_f, cfg, 41, "f")
+
+
+def test_fim_one_point_block(lib_f):
+    """A one-point block fs = fe of a 40 x 30 x 20 model: its value is the
+    internal field's at the nearest node, and the statistics are the internal
+    field's (many points), as for any block."""
+    cfg = structured_cfg(n=(1, 1, 1), fs=(7, 5, 3), ds=50.0, cl=(500.0, 400.0, 300.0), hurst=0.25, real=np.float32)
+    cfg["nc"] = (0.0, 0.0, 0.0)
+    cfg["fc"] = (39 * 50.0, 29 * 50.0, 19 * 50.0)
+    cfg = _fim(cfg)
+    _say("one-point block (7, 5, 3) of 40 x 30 x 20")
+    field, stats, _ = _check(lib_f, cfg, 19, "f")
+    assert field.size == 1 and np.isfinite(stats[2])
+
+
+def test_fim_points_outside_the_model_are_zero(lib_f):
+    """nc/fc given inside the points' extent: the internal grid covers the
+    model only, and points beyond it and its halo are 0 (in the reference no
+    process holds them and the field keeps its initial 0); the others match
+    the oracle."""
+    cfg = _fim(unstructured_cfg(npts=3000, box=(4000.0, 3000.0, 2000.0), dh=50.0, acf=0, hurst=0.25,
+                                cl=(600.0, 500.0, 400.0), sigma=0.08, seed=22, real=np.float32, nd=3,
+                                nc=(500.0, 400.0, 300.0), fc=(3000.0, 2400.0, 1500.0)), 0)
+    _say("points partly outside nc..fc")
+    field, _, ref = _check(lib_f, cfg, 17, "f")
+    zero = field == 0
+    print(f"  {int(zero.sum())} of {field.size} points outside", flush=True)
+    assert zero.sum() > 300 and np.array_equal(zero, ref["field"] == 0)
