@@ -817,25 +817,53 @@ static int scarf_ensemble_execute_body(int n, const struct scarf_member *members
         c.rescale = a.rescale;
         c.nharm = a.nharm > 0 ? a.nharm : s.nharm;
     }
-    // host draws, one thread per member at a time
+    // Host draws. Every member's setup first (one per thread), then its
+    // expected draw cost, nharm over the rejection test's acceptance
+    // (cnst * intg(kmin, kmax) / (kmax - kmin)). The members that would hold
+    // up the rest (cost above a share of the total) are drawn one after the
+    // other with every thread of the sampler, the others one per thread,
+    // costliest first. The draws are the exact streams either way.
     std::vector<scarf::SrmSetup> setups(n);
     std::vector<std::vector<double>> records(n);
     std::vector<int> nps(n, 0);
-    std::vector<double> host_s(n, 0.0);
+    std::vector<double> host_s(n, 0.0), cost(n, 0.0);
     std::vector<std::string> errs(n);
-    std::atomic<int> next{0};
-    auto draw_member = [&](int m) {
+    auto guard = [&](int m, auto &&f) {
+        try {
+            f();
+        } catch (const std::exception &ex) {
+            errs[m] = std::string("internal error: ") + ex.what();
+        } catch (...) {
+            errs[m] = "internal error";
+        }
+    };
+    auto on_members = [&](const std::vector<int> &list, auto &&f) {
+        std::atomic<int> next{0};
+        const std::function<void(int)> worker = [&](int) {
+            for (int q = next++; q < static_cast<int>(list.size()); q = next++) guard(list[q], [&] { f(list[q]); });
+        };
+        member_pool().run(worker);
+    };
+    std::vector<int> all(n);
+    for (int m = 0; m < n; ++m) all[m] = m;
+    on_members(all, [&](int m) {
         const auto t0 = std::chrono::steady_clock::now();
         std::string e;
         const int rc = kDouble ? scarf::srm_setup<double>(cfgs[m], setups[m], e)
                                : scarf::srm_setup<float>(cfgs[m], setups[m], e);
-        if (rc != 0) {
-            errs[m] = e;
-            return;
-        }
-        // members are drawn in parallel already: one thread per member's draw
-        const scarf::HarmonicDraw d = kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed, 1)
-                                              : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed, 1);
+        if (rc != 0) errs[m] = e;
+        const scarf::SrmSetup &su = setups[m];
+        const double p = su.kmax > su.kmin ? su.cnst * su.band / (su.kmax - su.kmin) : 1.0;
+        cost[m] = cfgs[m].nharm / std::min(1.0, std::max(p, 1e-6));
+        host_s[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+    for (int m = 0; m < n; ++m)
+        if (!errs[m].empty()) return fail("member " + std::to_string(m) + ": " + errs[m]);
+    auto draw_member = [&](int m, int threads) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const scarf::HarmonicDraw d =
+            kDouble ? scarf::srm_draw<double>(cfgs[m], setups[m], members[m].seed, threads)
+                    : scarf::srm_draw<float>(cfgs[m], setups[m], members[m].seed, threads);
         const scarf::HarmonicTable t = scarf::build_table(setups[m], d);
         const int np_ = std::max(64, (t.count() + 63) / 64 * 64);
         std::vector<double> &r = records[m];
@@ -848,20 +876,22 @@ static int scarf_ensemble_execute_body(int n, const struct scarf_member *members
             r[4 * size_t(np_) + l] = t.amp[l];
         }
         nps[m] = np_;
-        host_s[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        host_s[m] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    const std::function<void(int)> worker = [&](int) {
-        for (int m = next++; m < n; m = next++) {
-            try {
-                draw_member(m);
-            } catch (const std::exception &ex) {
-                errs[m] = std::string("internal error: ") + ex.what();
-            } catch (...) {
-                errs[m] = "internal error";
-            }
+    std::vector<int> order = all;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    const int threads = member_pool().size();
+    double total = 0.0;
+    for (int m = 0; m < n; ++m) total += cost[m];
+    std::vector<int> light;
+    for (int m : order) {
+        if (threads > 1 && cost[m] > total / (2.0 * threads)) {
+            guard(m, [&] { draw_member(m, threads); });  // the sampler's own workers
+        } else {
+            light.push_back(m);
         }
-    };
-    member_pool().run(worker);
+    }
+    on_members(light, [&](int m) { draw_member(m, 1); });
     for (int m = 0; m < n; ++m)
         if (!errs[m].empty()) return fail("member " + std::to_string(m) + ": " + errs[m]);
     if (!ensure_device()) return -1;
