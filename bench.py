@@ -563,7 +563,7 @@ def main(argv=None) -> int:
         r.setdefault("gpt_harm_per_s", _sig(rate / 1e9))
         if "direct_f32" in res and k != "direct_f32":
             r["speedup_vs_fp32_direct"] = _sig(res["direct_f32"]["field_ms"] / r["field_ms"])
-        # the rooflines of north_star: the reference formulation's 10 FP32 flops per
+        # two rooflines: the reference formulation's 10 FP32 flops per
         # point-harmonic against the FP32 vector peak, and the executed MFMA flops
         # (4 per product per point-harmonic) against dense bf16/fp16
         r["frac_fp32_valu"] = _sig(rate * 10 / (PEAK_VALU_TFLOPS * 1e12))
